@@ -1,0 +1,161 @@
+/*
+ * tangerine_gpu.h -- C ABI of the MI355X CSG signed-distance path.
+ *
+ * Drop-in for the evaluation path of Tangerine's SDFNode tree: trees are
+ * built with the same factories and transforms the Lua API calls, flattened
+ * to the reference bytecode, and evaluated on the GPU over point lists, dense
+ * grids (with naive surface nets) and camera rays.
+ *
+ * Conventions
+ *   - Every call returns TG_OK (0) or a negative TG_ERR_* code; on failure
+ *     tg_last_error() returns a message for the calling thread.  Builders
+ *     return NULL on failure.  The library never aborts the process.
+ *   - Host buffers are caller-owned.  Opaque handles are released with the
+ *     matching *_release / *_destroy call.
+ *   - Trees (tg_node) are immutable and may be shared between threads; every
+ *     transform returns a new tree, as the Lua bindings copy before they
+ *     transform (tangerine/lua_sdf.cpp:56-170).
+ *   - A tg_mesher or tg_tracer owns one HIP stream and its device buffers;
+ *     calls on different handles may run concurrently, calls on one handle
+ *     must be serialised by the caller.
+ *   - Plain pointers and sizes only; no C++ or framework types cross the ABI.
+ *
+ * Reference interfaces replaced (file:line in Aeva/tangerine):
+ *   builders        SDF::Sphere ... SDF::Flate       tangerine/sdf_evaluator.h:223-264,
+ *                   stale C ABI Make*Brush/Make*Op   tangerine/c_sdf.cpp:118-178
+ *   transforms      SDFNode::Move/Rotate/Scale       tangerine/sdf_evaluator.h:189-193,
+ *                   MoveTree/RotateTree              tangerine/c_sdf.cpp:86-98
+ *   compile         SDFNode::Compile/ProgramBuffer   tangerine/sdf_evaluator.h:110-163,185
+ *   point eval      EvalTree / SDFNode::Eval         tangerine/c_sdf.cpp:30-34, sdf_evaluator.h:175
+ *   gradient        SDFNode::Gradient                tangerine/sdf_evaluator.h:187
+ *   ray march       RayMarchTree / SDFNode::RayMarch tangerine/c_sdf.cpp:56-67, sdf_evaluator.h:210
+ *   grid export     MeshExportThread + par_surface_nets
+ *                                                   tangerine/export.cpp:320-381,
+ *                                                   third_party/naive-surface-nets/include/surface_nets.h:62-79
+ */
+#ifndef TANGERINE_GPU_H
+#define TANGERINE_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TG_OK 0
+#define TG_ERR_INVALID (-1)  /* bad argument */
+#define TG_ERR_DEVICE (-2)   /* HIP runtime failure or no device */
+#define TG_ERR_PROGRAM (-3)  /* malformed program words */
+#define TG_ERR_NOMEM (-4)    /* device allocation failed */
+#define TG_ERR_LIMIT (-5)    /* size beyond a documented limit */
+
+/* Evaluation flags. */
+#define TG_FLAG_FAST 1u      /* contracted FP32 numerics; default is bit parity with SDFInterpreter::Eval */
+#define TG_FLAG_NO_PRUNE 2u  /* interpret the whole program in every tile (no per-brick pruning) */
+
+typedef struct tg_node tg_node;
+typedef struct tg_program tg_program;
+typedef struct tg_mesher tg_mesher;
+
+/* Same fields and meaning as isosurface::regular_grid_t
+ * (third_party/naive-surface-nets/include/regular_grid.h): origin, cell
+ * size and cell counts; samples are (s+1) per axis at origin + i*d. */
+typedef struct tg_grid
+{
+	float x, y, z;
+	float dx, dy, dz;
+	uint64_t sx, sy, sz;
+} tg_grid;
+
+typedef struct tg_program_info
+{
+	uint64_t instructions;     /* device records, including Stop */
+	uint32_t max_stack;        /* interpreter stack high-water mark */
+	uint32_t brushes;
+	uint32_t set_ops;
+	uint32_t lipschitz;        /* 1 if every primitive is a distance bound (no ellipsoid) */
+	double flops_per_eval;     /* algorithmic FP32 work of one full-program evaluation */
+} tg_program_info;
+
+typedef struct tg_export_stats
+{
+	double eval_ms;            /* grid evaluation kernels (HIP events) */
+	double mesh_ms;            /* surface-nets kernels (HIP events) */
+	double total_ms;           /* whole export on the device stream */
+	uint64_t samples;          /* (sx+1)(sy+1)(sz+1) */
+	uint64_t tiles_evaluated;  /* 8^3-sample tiles interpreted */
+	uint64_t exact_evals;      /* samples actually interpreted */
+	double flops;              /* algorithmic FP32 work executed (cost table) */
+} tg_export_stats;
+
+/* ---- errors / device ----------------------------------------------------- */
+const char* tg_last_error(void);
+int tg_device_init(int device);
+int tg_device_synchronize(void);
+
+/* ---- tree builders (SDF::* factories; radii and half extents) -------------- */
+tg_node* tg_sphere(float radius);                                        /* sdf_evaluator.cpp:1206 */
+tg_node* tg_ellipsoid(float rx, float ry, float rz);                     /* :1216 */
+tg_node* tg_box(float ex, float ey, float ez);                           /* :1227 */
+tg_node* tg_torus(float major_radius, float minor_radius);               /* :1238 */
+tg_node* tg_cylinder(float radius, float half_height);                   /* :1250 */
+tg_node* tg_plane(float nx, float ny, float nz);                         /* :1260 */
+tg_node* tg_cone(float radius, float height);                            /* :1296 */
+tg_node* tg_coninder(float radius_low, float radius_high, float height); /* :1307 */
+/* Set operators share their operands, as SDF::Union(LHS, RHS) does (:1320-1354). */
+tg_node* tg_union(const tg_node* lhs, const tg_node* rhs);
+tg_node* tg_inter(const tg_node* lhs, const tg_node* rhs);
+tg_node* tg_diff(const tg_node* lhs, const tg_node* rhs);
+tg_node* tg_blend_union(float threshold, const tg_node* lhs, const tg_node* rhs);
+tg_node* tg_blend_inter(float threshold, const tg_node* lhs, const tg_node* rhs);
+tg_node* tg_blend_diff(float threshold, const tg_node* lhs, const tg_node* rhs);
+tg_node* tg_flate(const tg_node* node, float radius);                    /* :1356 */
+/* Transforms return a transformed copy. */
+tg_node* tg_move(const tg_node* node, float x, float y, float z);
+tg_node* tg_rotate(const tg_node* node, float qx, float qy, float qz, float qw);
+tg_node* tg_rotate_x(const tg_node* node, float degrees);                /* SDF::RotateX :1180 */
+tg_node* tg_rotate_y(const tg_node* node, float degrees);
+tg_node* tg_rotate_z(const tg_node* node, float degrees);
+tg_node* tg_scale(const tg_node* node, float scale);
+void tg_node_release(tg_node* node);                                     /* DiscardTree c_sdf.cpp:70 */
+int tg_node_bounds(const tg_node* node, float min_max[6]);               /* SDFNode::Bounds */
+int tg_node_has_finite_bounds(const tg_node* node);                      /* TreeHasFiniteBounds c_sdf.cpp:77 */
+int tg_node_leaf_count(const tg_node* node);                             /* SDFNode::LeafCount */
+int tg_node_stack_size(const tg_node* node);                             /* SDFNode::StackSize */
+/* Reference ProgramBuffer words terminated by Stop (SDFInterpreter ctor,
+ * sdf_evaluator.cpp:1376-1382).  Writes at most `capacity` words and sets
+ * *count to the full length. */
+int tg_node_compile(const tg_node* node, uint32_t* words, uint64_t capacity, uint64_t* count);
+
+/* ---- device programs ------------------------------------------------------ */
+int tg_program_from_node(const tg_node* node, tg_program** out);
+int tg_program_from_words(const uint32_t* words, uint64_t count, tg_program** out);
+int tg_program_info_get(const tg_program* program, tg_program_info* out);
+void tg_program_destroy(tg_program* program);
+
+/* ---- point queries (EvalTree / SDFNode::Eval, batched) --------------------- */
+/* xyz: 3 floats per point; host buffers. */
+int tg_eval_points(const tg_program* program, const float* xyz, uint64_t count, float* out, uint32_t flags);
+/* Same with device buffers on `stream` (a hipStream_t, or NULL for the null stream). */
+int tg_eval_points_device(const tg_program* program, const float* xyz, uint64_t count, float* out,
+                          uint32_t flags, void* stream);
+
+/* ---- grid export (MeshExportThread shape: evaluate, then naive surface nets) */
+int tg_mesher_create(tg_mesher** out);
+void tg_mesher_destroy(tg_mesher* mesher);
+int tg_mesher_export(tg_mesher* mesher, const tg_program* program, const tg_grid* grid, uint32_t flags);
+/* Mesh caller-supplied samples ((sx+1)(sy+1)(sz+1) floats, x fastest). */
+int tg_mesher_mesh_values(tg_mesher* mesher, const float* values, const tg_grid* grid);
+int tg_mesher_counts(const tg_mesher* mesher, uint64_t* vertices, uint64_t* triangles);
+/* vertices: 3 floats each; triangles: 3 uint32 vertex ids each. */
+int tg_mesher_copy_mesh(const tg_mesher* mesher, float* vertices, uint32_t* triangles);
+/* Samples of the last export, x fastest. */
+int tg_mesher_copy_grid(const tg_mesher* mesher, float* values);
+int tg_mesher_stats(const tg_mesher* mesher, tg_export_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TANGERINE_GPU_H */

@@ -1,0 +1,373 @@
+// Test-infrastructure only: a C-ABI harness around the reference's own
+// CSG / octree / surface-nets sources, compiled from /root/reference by
+// oracle/Makefile into oracle/_ref/libtgref.so.  It is used to generate the
+// golden vectors under tests/golden/ and, optionally, as the timed CPU
+// baseline ("kind": "reference").  Nothing in the product links it.
+//
+// Every entry point forwards to the reference implementation unchanged:
+//   builders      -> SDF::* factories            tangerine/sdf_evaluator.cpp:1169-1372
+//   transforms    -> Lua copy-then-transform     tangerine/lua_sdf.cpp:56-170
+//   eval          -> SDFNode::Eval               tangerine/sdf_evaluator.cpp:463,774,1059
+//   interp_eval   -> SDFInterpreter::Eval        tangerine/sdf_evaluator.cpp:1376-1605
+//   clip          -> SDFNode::Clip               tangerine/sdf_evaluator.cpp:468,782,1064
+//   octree        -> SDFOctree                   tangerine/sdf_evaluator.cpp:1609-2005
+//   gradient      -> SDFNode::Gradient           tangerine/sdf_evaluator.cpp:298-333
+//   raymarch      -> SDFNode::RayMarch           tangerine/sdf_evaluator.cpp:336-353
+//   surface nets  -> isosurface::surface_nets    third_party/naive-surface-nets/src/surface_nets.cpp:30-580
+//   threaded export -> AsyncParallelSurfaceNets thunks (surface_nets.cpp:829-1122)
+//                      driven like MeshExportThread (tangerine/export.cpp:320-381)
+//                      on a pool of max(hw-1,2)+1 threads (tangerine/scheduler.cpp:141-148)
+//   pool grid fill  -> Pool (tangerine/threadpool.cpp:27-40), loop shape of
+//                      tangerine/export.cpp:401-427, SDFOctree::Eval(Exact)
+
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "sdf_evaluator.h"
+#include "surface_nets.h"
+#include "threadpool.h"
+
+using namespace glm;
+
+namespace {
+
+struct Handle
+{
+	SDFNodeShared Node;
+};
+
+struct Octree
+{
+	SDFOctreeShared Tree;
+};
+
+struct Mesh
+{
+	isosurface::mesh M;
+	uint64_t Evals = 0;
+};
+
+Handle* Wrap(SDFNodeShared Node)
+{
+	if (!Node)
+	{
+		return nullptr;
+	}
+	return new Handle{ Node };
+}
+
+isosurface::regular_grid_t MakeGrid(const float* Origin, const float* Step, const uint64_t* Count)
+{
+	isosurface::regular_grid_t Grid;
+	Grid.x = Origin[0];
+	Grid.y = Origin[1];
+	Grid.z = Origin[2];
+	Grid.dx = Step[0];
+	Grid.dy = Step[1];
+	Grid.dz = Step[2];
+	Grid.sx = Count[0];
+	Grid.sy = Count[1];
+	Grid.sz = Count[2];
+	return Grid;
+}
+
+}  // namespace
+
+extern "C" {
+
+void* tgref_sphere(float R) { return Wrap(SDF::Sphere(R)); }
+void* tgref_ellipsoid(float X, float Y, float Z) { return Wrap(SDF::Ellipsoid(X, Y, Z)); }
+void* tgref_box(float X, float Y, float Z) { return Wrap(SDF::Box(X, Y, Z)); }
+void* tgref_torus(float Major, float Minor) { return Wrap(SDF::Torus(Major, Minor)); }
+void* tgref_cylinder(float R, float H) { return Wrap(SDF::Cylinder(R, H)); }
+void* tgref_plane(float X, float Y, float Z) { return Wrap(SDF::Plane(X, Y, Z)); }
+void* tgref_cone(float R, float H) { return Wrap(SDF::Cone(R, H)); }
+void* tgref_coninder(float L, float H, float Height) { return Wrap(SDF::Coninder(L, H, Height)); }
+
+// Set operators take the operands as the Lua bindings do (no copy, lua_sdf.cpp:348-383).
+void* tgref_op(int Kind, float Threshold, void* A, void* B)
+{
+	SDFNodeShared& L = ((Handle*)A)->Node;
+	SDFNodeShared& R = ((Handle*)B)->Node;
+	switch (Kind)
+	{
+	case 0: return Wrap(SDF::Union(L, R));
+	case 1: return Wrap(SDF::Inter(L, R));
+	case 2: return Wrap(SDF::Diff(L, R));
+	case 3: return Wrap(SDF::BlendUnion(Threshold, L, R));
+	case 4: return Wrap(SDF::BlendInter(Threshold, L, R));
+	case 5: return Wrap(SDF::BlendDiff(Threshold, L, R));
+	default: return nullptr;
+	}
+}
+
+void* tgref_flate(void* A, float Radius)
+{
+	return Wrap(SDF::Flate(((Handle*)A)->Node, Radius));
+}
+
+// Transforms copy first, as every Lua transform binding does (lua_sdf.cpp:56-170).
+void* tgref_move(void* A, float X, float Y, float Z)
+{
+	SDFNodeShared N = ((Handle*)A)->Node->Copy();
+	N->Move(vec3(X, Y, Z));
+	return Wrap(N);
+}
+
+void* tgref_rotate(void* A, float X, float Y, float Z, float W)
+{
+	SDFNodeShared N = ((Handle*)A)->Node->Copy();
+	N->Rotate(quat(W, X, Y, Z));
+	return Wrap(N);
+}
+
+void* tgref_rotate_axis(void* A, int Axis, float Degrees)
+{
+	SDFNodeShared N = ((Handle*)A)->Node->Copy();
+	if (Axis == 0) SDF::RotateX(N, Degrees);
+	else if (Axis == 1) SDF::RotateY(N, Degrees);
+	else SDF::RotateZ(N, Degrees);
+	return Wrap(N);
+}
+
+void* tgref_scale(void* A, float S)
+{
+	SDFNodeShared N = ((Handle*)A)->Node->Copy();
+	N->Scale(S);
+	return Wrap(N);
+}
+
+void tgref_free(void* A) { delete (Handle*)A; }
+
+int tgref_leaf_count(void* A) { return ((Handle*)A)->Node->LeafCount(); }
+uint64_t tgref_stack_size(void* A) { return ((Handle*)A)->Node->StackSize; }
+
+void tgref_bounds(void* A, float* Out)
+{
+	AABB B = ((Handle*)A)->Node->Bounds();
+	Out[0] = B.Min.x; Out[1] = B.Min.y; Out[2] = B.Min.z;
+	Out[3] = B.Max.x; Out[4] = B.Max.y; Out[5] = B.Max.z;
+}
+
+void tgref_eval(void* A, const float* P, uint64_t N, float* Out)
+{
+	SDFNode* Node = ((Handle*)A)->Node.get();
+	for (uint64_t i = 0; i < N; ++i)
+	{
+		Out[i] = Node->Eval(vec3(P[3 * i], P[3 * i + 1], P[3 * i + 2]));
+	}
+}
+
+uint64_t tgref_compile(void* A, uint32_t* Words, uint64_t Capacity)
+{
+	SDFInterpreter Interp(((Handle*)A)->Node);
+	uint64_t N = Interp.Program.Size();
+	for (uint64_t i = 0; i < N && i < Capacity; ++i)
+	{
+		std::memcpy(&Words[i], &Interp.Program.Words[i], 4);
+	}
+	return N;
+}
+
+void tgref_interp_eval(void* A, const float* P, uint64_t N, float* Out)
+{
+	SDFInterpreter Interp(((Handle*)A)->Node);
+	for (uint64_t i = 0; i < N; ++i)
+	{
+		Out[i] = Interp.Eval(vec3(P[3 * i], P[3 * i + 1], P[3 * i + 2]));
+	}
+}
+
+void* tgref_clip(void* A, float X, float Y, float Z, float R)
+{
+	return Wrap(((Handle*)A)->Node->Clip(vec3(X, Y, Z), R));
+}
+
+void tgref_gradient(void* A, const float* P, uint64_t N, float* Out)
+{
+	SDFNode* Node = ((Handle*)A)->Node.get();
+	for (uint64_t i = 0; i < N; ++i)
+	{
+		vec3 G = Node->Gradient(vec3(P[3 * i], P[3 * i + 1], P[3 * i + 2]));
+		Out[3 * i] = G.x; Out[3 * i + 1] = G.y; Out[3 * i + 2] = G.z;
+	}
+}
+
+void tgref_raymarch(void* A, const float* Start, const float* Dir, uint64_t N, int Iterations, float Epsilon,
+	uint8_t* Hit, float* Travel, float* Position)
+{
+	SDFNode* Node = ((Handle*)A)->Node.get();
+	for (uint64_t i = 0; i < N; ++i)
+	{
+		RayHit R = Node->RayMarch(
+			vec3(Start[3 * i], Start[3 * i + 1], Start[3 * i + 2]),
+			vec3(Dir[3 * i], Dir[3 * i + 1], Dir[3 * i + 2]), Iterations, Epsilon);
+		Hit[i] = R.Hit ? 1 : 0;
+		Travel[i] = R.Travel;
+		Position[3 * i] = R.Position.x; Position[3 * i + 1] = R.Position.y; Position[3 * i + 2] = R.Position.z;
+	}
+}
+
+void* tgref_octree_create(void* A, float TargetSize, int Coalesce, int MaxDepth, float Margin)
+{
+	SDFOctreeShared T = SDFOctree::Create(((Handle*)A)->Node, TargetSize, Coalesce != 0, MaxDepth, Margin);
+	if (!T) return nullptr;
+	return new Octree{ T };
+}
+
+void tgref_octree_free(void* O) { delete (Octree*)O; }
+
+void tgref_octree_eval(void* O, const float* P, uint64_t N, int Exact, float* Out)
+{
+	SDFOctree* T = ((Octree*)O)->Tree.get();
+	for (uint64_t i = 0; i < N; ++i)
+	{
+		Out[i] = T->Eval(vec3(P[3 * i], P[3 * i + 1], P[3 * i + 2]), Exact != 0);
+	}
+}
+
+// Dense grid fill with SDFOctree::Eval(Exact) on the reference Pool: every
+// Pool thread claims sample indices from one atomic counter, as in
+// export.cpp:401-427.  Samples are (Count+1) per axis, x fastest.
+// Only z-slabs [K0, K1) are filled (for bounded CPU timing).
+int tgref_pool_grid_eval(void* O, const float* Origin, const float* Step, const uint64_t* Count,
+	uint64_t K0, uint64_t K1, float* Out)
+{
+	SDFOctree* T = ((Octree*)O)->Tree.get();
+	const uint64_t NX = Count[0] + 1, NY = Count[1] + 1;
+	const uint64_t Slice = NX * NY;
+	const uint64_t Total = Slice * (K1 - K0);
+	std::atomic<uint64_t> Next(0);
+	Pool([&]()
+	{
+		for (;;)
+		{
+			uint64_t n = Next.fetch_add(1);
+			if (n >= Total) break;
+			uint64_t k = n / Slice + K0;
+			uint64_t j = (n % Slice) / NX;
+			uint64_t i = n % NX;
+			float X = Origin[0] + float(i) * Step[0];
+			float Y = Origin[1] + float(j) * Step[1];
+			float Z = Origin[2] + float(k) * Step[2];
+			Out[n] = T->Eval(vec3(X, Y, Z), true);
+		}
+	});
+	return std::max(int(std::thread::hardware_concurrency()), 2);
+}
+
+// Serial reference surface nets over precomputed sample values (identical-grid
+// topology checks).  Values are (Count+1) per axis, x fastest.
+void* tgref_surface_nets_values(const float* Values, const float* Origin, const float* Step, const uint64_t* Count)
+{
+	isosurface::regular_grid_t Grid = MakeGrid(Origin, Step, Count);
+	const uint64_t NX = Count[0] + 1, NY = Count[1] + 1;
+	auto Lookup = [&](float X, float Y, float Z) -> float
+	{
+		uint64_t i = (uint64_t)std::llround((X - Grid.x) / Grid.dx);
+		uint64_t j = (uint64_t)std::llround((Y - Grid.y) / Grid.dy);
+		uint64_t k = (uint64_t)std::llround((Z - Grid.z) / Grid.dz);
+		return Values[i + NX * (j + NY * k)];
+	};
+	Mesh* M = new Mesh;
+	std::atomic_bool Live(true);
+	isosurface::surface_nets(Lookup, Grid, M->M, Live, 0.f);
+	return M;
+}
+
+// Threaded export in the shape of MeshExportThread: octree (TargetSize .25)
+// evaluation behind the surface-nets thunks, vertex loop over z-slabs
+// [K0, K1) then face loop, both spread over Threads workers claiming work
+// items from atomic counters.  Threads == 0 picks max(hw-1,2)+1.
+void* tgref_export_threaded(void* O, const float* Origin, const float* Step, const uint64_t* Count,
+	uint64_t K0, uint64_t K1, int Threads)
+{
+	SDFOctree* T = ((Octree*)O)->Tree.get();
+	std::atomic<uint64_t> Evals(0);
+	isosurface::AsyncParallelSurfaceNets Task;
+	Task.Grid = MakeGrid(Origin, Step, Count);
+	Task.ImplicitFunction = [&](float X, float Y, float Z) -> float
+	{
+		return T->Eval(vec3(X, Y, Z));
+	};
+	Task.Setup();
+	if (Threads <= 0)
+	{
+		Threads = std::max(int(std::thread::hardware_concurrency()) - 1, 2) + 1;
+	}
+	std::vector<uint64_t> Slabs;
+	for (uint64_t k = K0; k < K1 && k < Task.FirstLoopDomain.size(); ++k) Slabs.push_back(k);
+	{
+		std::atomic<uint64_t> Next(0);
+		std::vector<std::thread> Workers;
+		for (int t = 0; t < Threads; ++t)
+		{
+			Workers.emplace_back([&]()
+			{
+				for (;;)
+				{
+					uint64_t n = Next.fetch_add(1);
+					if (n >= Slabs.size()) break;
+					Task.FirstLoopThunk(Task, Slabs[n]);
+				}
+			});
+		}
+		for (auto& W : Workers) W.join();
+	}
+	std::vector<std::pair<const std::size_t, std::uint64_t>*> Items;
+	for (auto& KV : Task.SecondLoopDomain) Items.push_back(&KV);
+	{
+		std::atomic<uint64_t> Next(0);
+		std::vector<std::thread> Workers;
+		for (int t = 0; t < Threads; ++t)
+		{
+			Workers.emplace_back([&]()
+			{
+				for (;;)
+				{
+					uint64_t n = Next.fetch_add(1);
+					if (n >= Items.size()) break;
+					Task.SecondLoopThunk(Task, *Items[n]);
+				}
+			});
+		}
+		for (auto& W : Workers) W.join();
+	}
+	Mesh* M = new Mesh;
+	M->M = Task.OutputMesh;
+	const uint64_t NX = Task.Grid.sx, NY = Task.Grid.sy;
+	M->Evals = 8 * NX * NY * Slabs.size() + 6 * Items.size();
+	return M;
+}
+
+int tgref_threads_default() { return std::max(int(std::thread::hardware_concurrency()) - 1, 2) + 1; }
+int tgref_hardware_concurrency() { return int(std::thread::hardware_concurrency()); }
+
+uint64_t tgref_mesh_vertex_count(void* M) { return ((Mesh*)M)->M.vertices_.size(); }
+uint64_t tgref_mesh_face_count(void* M) { return ((Mesh*)M)->M.faces_.size(); }
+uint64_t tgref_mesh_evals(void* M) { return ((Mesh*)M)->Evals; }
+
+void tgref_mesh_copy(void* M, float* Vertices, uint64_t* Faces)
+{
+	Mesh* Ms = (Mesh*)M;
+	for (size_t i = 0; i < Ms->M.vertices_.size(); ++i)
+	{
+		Vertices[3 * i] = Ms->M.vertices_[i].x;
+		Vertices[3 * i + 1] = Ms->M.vertices_[i].y;
+		Vertices[3 * i + 2] = Ms->M.vertices_[i].z;
+	}
+	for (size_t i = 0; i < Ms->M.faces_.size(); ++i)
+	{
+		Faces[3 * i] = Ms->M.faces_[i].v0;
+		Faces[3 * i + 1] = Ms->M.faces_[i].v1;
+		Faces[3 * i + 2] = Ms->M.faces_[i].v2;
+	}
+}
+
+void tgref_mesh_free(void* M) { delete (Mesh*)M; }
+
+}  // extern "C"

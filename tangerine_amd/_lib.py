@@ -1,0 +1,134 @@
+"""ctypes binding of libtangerine_gpu.so (include/tangerine_gpu.h).
+
+The library is the product: there is no CPU fallback.  Loading fails loudly
+when the shared object is missing; device calls fail with the library's own
+error message when no GPU is present.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libtangerine_gpu.so"
+
+TG_OK = 0
+TG_FLAG_FAST = 1
+TG_FLAG_NO_PRUNE = 2
+
+_f = C.c_float
+_u32 = C.c_uint32
+_u64 = C.c_uint64
+_p = C.c_void_p
+_fp = C.POINTER(C.c_float)
+_u32p = C.POINTER(C.c_uint32)
+_u64p = C.POINTER(C.c_uint64)
+
+
+class TgGrid(C.Structure):
+    _fields_ = [("x", _f), ("y", _f), ("z", _f), ("dx", _f), ("dy", _f), ("dz", _f),
+                ("sx", _u64), ("sy", _u64), ("sz", _u64)]
+
+
+class TgProgramInfo(C.Structure):
+    _fields_ = [("instructions", _u64), ("max_stack", _u32), ("brushes", _u32), ("set_ops", _u32),
+                ("lipschitz", _u32), ("flops_per_eval", C.c_double)]
+
+
+class TgExportStats(C.Structure):
+    _fields_ = [("eval_ms", C.c_double), ("mesh_ms", C.c_double), ("total_ms", C.c_double),
+                ("samples", _u64), ("tiles_evaluated", _u64), ("exact_evals", _u64), ("flops", C.c_double)]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "tg_last_error": (C.c_char_p, []),
+    "tg_device_init": (C.c_int, [C.c_int]),
+    "tg_device_synchronize": (C.c_int, []),
+    "tg_sphere": (_p, [_f]),
+    "tg_ellipsoid": (_p, [_f, _f, _f]),
+    "tg_box": (_p, [_f, _f, _f]),
+    "tg_torus": (_p, [_f, _f]),
+    "tg_cylinder": (_p, [_f, _f]),
+    "tg_plane": (_p, [_f, _f, _f]),
+    "tg_cone": (_p, [_f, _f]),
+    "tg_coninder": (_p, [_f, _f, _f]),
+    "tg_union": (_p, [_p, _p]),
+    "tg_inter": (_p, [_p, _p]),
+    "tg_diff": (_p, [_p, _p]),
+    "tg_blend_union": (_p, [_f, _p, _p]),
+    "tg_blend_inter": (_p, [_f, _p, _p]),
+    "tg_blend_diff": (_p, [_f, _p, _p]),
+    "tg_flate": (_p, [_p, _f]),
+    "tg_move": (_p, [_p, _f, _f, _f]),
+    "tg_rotate": (_p, [_p, _f, _f, _f, _f]),
+    "tg_rotate_x": (_p, [_p, _f]),
+    "tg_rotate_y": (_p, [_p, _f]),
+    "tg_rotate_z": (_p, [_p, _f]),
+    "tg_scale": (_p, [_p, _f]),
+    "tg_node_release": (None, [_p]),
+    "tg_node_bounds": (C.c_int, [_p, _fp]),
+    "tg_node_has_finite_bounds": (C.c_int, [_p]),
+    "tg_node_leaf_count": (C.c_int, [_p]),
+    "tg_node_stack_size": (C.c_int, [_p]),
+    "tg_node_compile": (C.c_int, [_p, _u32p, _u64, _u64p]),
+    "tg_program_from_node": (C.c_int, [_p, C.POINTER(_p)]),
+    "tg_program_from_words": (C.c_int, [_u32p, _u64, C.POINTER(_p)]),
+    "tg_program_info_get": (C.c_int, [_p, C.POINTER(TgProgramInfo)]),
+    "tg_program_destroy": (None, [_p]),
+    "tg_eval_points": (C.c_int, [_p, _fp, _u64, _fp, _u32]),
+    "tg_eval_points_device": (C.c_int, [_p, _p, _u64, _p, _u32, _p]),
+    "tg_mesher_create": (C.c_int, [C.POINTER(_p)]),
+    "tg_mesher_destroy": (None, [_p]),
+    "tg_mesher_export": (C.c_int, [_p, _p, C.POINTER(TgGrid), _u32]),
+    "tg_mesher_mesh_values": (C.c_int, [_p, _fp, C.POINTER(TgGrid)]),
+    "tg_mesher_counts": (C.c_int, [_p, _u64p, _u64p]),
+    "tg_mesher_copy_mesh": (C.c_int, [_p, _fp, _u32p]),
+    "tg_mesher_copy_grid": (C.c_int, [_p, _fp]),
+    "tg_mesher_stats": (C.c_int, [_p, C.POINTER(TgExportStats)]),
+}
+
+
+class TangerineError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load(path: os.PathLike | str | None = None) -> C.CDLL:
+    """Loads the shared library (once) and declares every signature."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise TangerineError(
+            f"{p} is missing: build the HIP library first (python -m tangerine_amd.build)")
+    lib = C.CDLL(str(p))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    msg = load().tg_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(rc: int, what: str) -> None:
+    if rc != TG_OK:
+        raise TangerineError(f"{what} failed ({rc}): {last_error()}")
+
+
+def grid_struct(origin, step, cells) -> TgGrid:
+    g = TgGrid()
+    g.x, g.y, g.z = (float(v) for v in origin)
+    g.dx, g.dy, g.dz = (float(v) for v in step)
+    g.sx, g.sy, g.sz = (int(v) for v in cells)
+    return g

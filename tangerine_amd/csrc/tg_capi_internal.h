@@ -1,0 +1,38 @@
+// Internal pieces shared by the C-ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <string>
+
+#include "tg_program.h"
+
+namespace tg {
+
+extern thread_local std::string g_last_error;
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+int ensure_device();
+
+// Grow-only device allocation.
+struct DevBuf
+{
+	void* ptr = nullptr;
+	size_t bytes = 0;
+	int ensure(size_t need);
+	void release();
+	template <typename T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+}  // namespace tg
+
+struct tg_program
+{
+	tg::DecodedProgram decoded;
+	tg::DevBuf device;  // Instr records
+};
+
+namespace tg {
+int upload_program(const DecodedProgram& d, tg_program** out);
+}

@@ -1,0 +1,43 @@
+// Device surface-nets stages (see tg_mesh.hip for the data layout).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tg {
+
+struct MeshArgs
+{
+	// grid (same meaning as isosurface::regular_grid_t)
+	float gx, gy, gz, dx, dy, dz;
+	uint32_t sx, sy, sz;          // cubes per axis
+	uint32_t nx, ny, nz;          // samples per axis (= s + 1)
+	uint32_t tx, ty, tz;          // sample tiles per axis
+	uint32_t cbx, cby, cbz;       // cube bricks per axis (8^3 cubes each)
+	// inputs
+	const float* pool;
+	const uint32_t* tile_slot;    // null: identity
+	const uint8_t* signs;
+	uint32_t row_bytes;
+	const uint8_t* brick_flags;   // null: every cube brick may hold active cubes
+	// intermediates
+	uint32_t aw;                  // 64-bit words per cube row
+	uint64_t* active;
+	uint32_t* word_count;
+	uint32_t* word_base;
+	uint32_t* quad_count;
+	uint32_t* quad_base;
+	// outputs
+	float* vertices;
+	uint32_t* triangles;
+};
+
+hipError_t mesh_classify(const MeshArgs& a, hipStream_t s);
+hipError_t mesh_vertices(const MeshArgs& a, hipStream_t s);
+hipError_t mesh_quad_count(const MeshArgs& a, hipStream_t s);
+hipError_t mesh_quad_write(const MeshArgs& a, hipStream_t s);
+hipError_t mesh_pool_to_dense(const MeshArgs& a, float* dense, uint32_t k0, uint32_t nk, hipStream_t s);
+hipError_t mesh_dense_to_pool(const MeshArgs& a, const float* dense, hipStream_t s);
+hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, void* temp, size_t& temp_bytes, hipStream_t s);
+
+}  // namespace tg

@@ -1,0 +1,346 @@
+// Device naive surface nets over a sampled distance grid.
+//
+// Follows the serial reference isosurface::surface_nets
+// (third_party/naive-surface-nets/src/surface_nets.cpp:30-580):
+//   * a cube (i,j,k), 0 <= i < sx, is active iff one of its 12 edges has
+//     corner signs that differ under `s >= 0` (:728-803);
+//   * its vertex is the mean of the edge crossings p1 + t (p2 - p1),
+//     t = (0 - s1) / (s2 - s1), accumulated in edge order 0..11 and mapped to
+//     world space by min + (max - min) * c / s (:258-296), all in float
+//     without contraction (this file is built with -ffp-contract=off);
+//   * vertex ids follow raster order of the cube index i + j sx + k sx sy,
+//     which is the serial reference's numbering (:91-93, :298-300);
+//   * each active cube with i, j, k > 0 emits, for the directed edges (0,4),
+//     (3,0), (0,1), a quad over the neighbour cubes {0,1,2}, {0,5,4},
+//     {2,3,4} when all three are active, ordered {0,1,2} if the edge's second
+//     value is greater than its first and {2,1,0} otherwise, split into the
+//     triangles (v0,v1,v2), (v0,v2,v3) (:384-556).  The reference emits faces
+//     in hash-map order; here they come in raster order of the cube, then edge
+//     order, then triangle order.
+//
+// Data layout (built by the evaluation kernels):
+//   pool  : 8^3-sample tiles, 512 floats each, [z][y][x] inside a tile; the
+//           slot of tile t is tile_slot[t] (identity when null).
+//   signs : one bit per sample (1 = s >= 0); row (j, k) of the sample grid is
+//           row_bytes bytes at (k * ny + j) * row_bytes, x bit-packed LSB first.
+//   active: one bit per cube; cube row (j, k) is `aw` 64-bit words at
+//           (k * sy + j) * aw.
+
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+#include <stdint.h>
+
+#include "tg_mesh.h"
+#include "tg_types.h"
+
+namespace tg {
+
+namespace {
+
+__device__ __forceinline__ float pool_at(const MeshArgs& a, uint32_t i, uint32_t j, uint32_t k)
+{
+	const uint32_t tile = (i >> 3) + a.tx * ((j >> 3) + a.ty * (k >> 3));
+	const uint32_t slot = a.tile_slot ? a.tile_slot[tile] : tile;
+	return a.pool[size_t(slot) * kTileSamples + ((k & 7) * 64 + (j & 7) * 8 + (i & 7))];
+}
+
+__device__ __forceinline__ uint64_t sign_word(const MeshArgs& a, uint32_t j, uint32_t k, uint32_t w)
+{
+	const uint64_t* row = reinterpret_cast<const uint64_t*>(a.signs + (size_t(k) * a.ny + j) * a.row_bytes);
+	return (w * 8u < a.row_bytes) ? row[w] : 0ull;
+}
+
+__device__ __forceinline__ bool cube_active(const MeshArgs& a, uint32_t i, uint32_t j, uint32_t k)
+{
+	const uint64_t word = a.active[(size_t(k) * a.sy + j) * a.aw + (i >> 6)];
+	return (word >> (i & 63)) & 1ull;
+}
+
+__device__ __forceinline__ uint32_t vertex_id(const MeshArgs& a, uint32_t i, uint32_t j, uint32_t k)
+{
+	const size_t wi = (size_t(k) * a.sy + j) * a.aw + (i >> 6);
+	const uint64_t below = a.active[wi] & ((1ull << (i & 63)) - 1ull);
+	return a.word_base[wi] + uint32_t(__popcll(below));
+}
+
+// Thread per (cube row, word): active bits and their count.
+__global__ void __launch_bounds__(256) k_classify(MeshArgs a)
+{
+	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	const uint64_t rows = uint64_t(a.sy) * a.sz;
+	if (t >= rows * a.aw) return;
+	const uint32_t w = uint32_t(t % a.aw);
+	const uint64_t row = t / a.aw;
+	const uint32_t j = uint32_t(row % a.sy), k = uint32_t(row / a.sy);
+	const uint64_t s0 = sign_word(a, j, k, w), s1 = sign_word(a, j + 1, k, w);
+	const uint64_t s2 = sign_word(a, j, k + 1, w), s3 = sign_word(a, j + 1, k + 1, w);
+	const uint64_t n0 = sign_word(a, j, k, w + 1), n1 = sign_word(a, j + 1, k, w + 1);
+	const uint64_t n2 = sign_word(a, j, k + 1, w + 1), n3 = sign_word(a, j + 1, k + 1, w + 1);
+	const uint64_t all0 = s0 & s1 & s2 & s3, any0 = s0 | s1 | s2 | s3;
+	const uint64_t all1 = n0 & n1 & n2 & n3, any1 = n0 | n1 | n2 | n3;
+	// Corner x = i and x = i + 1: shift the next-sample bits down by one.
+	const uint64_t all_hi = (all0 >> 1) | (all1 << 63);
+	const uint64_t any_hi = (any0 >> 1) | (any1 << 63);
+	uint64_t act = ~(all0 & all_hi) & (any0 | any_hi);
+	const uint64_t first = uint64_t(w) * 64;
+	if (first + 64 > a.sx)
+	{
+		const uint32_t keep = (a.sx > first) ? uint32_t(a.sx - first) : 0u;
+		act &= (keep >= 64) ? ~0ull : ((1ull << keep) - 1ull);
+	}
+	if (a.brick_flags)
+	{
+		// Only cube bricks (8^3 cubes) flagged for exact evaluation may hold
+		// active cubes; the flags hold one byte per brick.
+		const uint32_t bj = j >> 3, bk = k >> 3;
+		uint64_t m = 0;
+#pragma unroll
+		for (int b = 0; b < 8; ++b)
+		{
+			const uint32_t bi = w * 8 + b;
+			if (bi < a.cbx && a.brick_flags[bi + a.cbx * (bj + a.cby * bk)]) m |= 0xffull << (8 * b);
+		}
+		act &= m;
+	}
+	a.active[t] = act;
+	a.word_count[t] = uint32_t(__popcll(act));
+}
+
+// Thread per active word: vertex placement.
+__global__ void __launch_bounds__(256) k_vertices(MeshArgs a)
+{
+	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	const uint64_t rows = uint64_t(a.sy) * a.sz;
+	if (t >= rows * a.aw) return;
+	uint64_t act = a.active[t];
+	if (!act) return;
+	const uint32_t w = uint32_t(t % a.aw);
+	const uint64_t row = t / a.aw;
+	const uint32_t j = uint32_t(row % a.sy), k = uint32_t(row / a.sy);
+	uint32_t id = a.word_base[t];
+	const float maxx = a.gx + float(a.sx) * a.dx;
+	const float maxy = a.gy + float(a.sy) * a.dy;
+	const float maxz = a.gz + float(a.sz) * a.dz;
+	const float fsx = float(a.sx), fsy = float(a.sy), fsz = float(a.sz);
+	while (act)
+	{
+		const int b = __ffsll((long long)act) - 1;
+		act &= act - 1;
+		const uint32_t i = w * 64 + uint32_t(b);
+		float s[8];
+		s[0] = pool_at(a, i, j, k);
+		s[1] = pool_at(a, i + 1, j, k);
+		s[2] = pool_at(a, i + 1, j + 1, k);
+		s[3] = pool_at(a, i, j + 1, k);
+		s[4] = pool_at(a, i, j, k + 1);
+		s[5] = pool_at(a, i + 1, j, k + 1);
+		s[6] = pool_at(a, i + 1, j + 1, k + 1);
+		s[7] = pool_at(a, i, j + 1, k + 1);
+		// corner offsets (x, y, z) and the 12 edges of the reference cube numbering
+		constexpr uint8_t cx[8] = { 0, 1, 1, 0, 0, 1, 1, 0 };
+		constexpr uint8_t cy[8] = { 0, 0, 1, 1, 0, 0, 1, 1 };
+		constexpr uint8_t cz[8] = { 0, 0, 0, 0, 1, 1, 1, 1 };
+		constexpr uint8_t e0[12] = { 0, 1, 2, 3, 4, 5, 6, 7, 0, 1, 2, 3 };
+		constexpr uint8_t e1[12] = { 1, 2, 3, 0, 5, 6, 7, 4, 4, 5, 6, 7 };
+		const float fi = float(i), fj = float(j), fk = float(k);
+		float sx_ = 0.0f, sy_ = 0.0f, sz_ = 0.0f;
+		int n = 0;
+#pragma unroll
+		for (int e = 0; e < 12; ++e)
+		{
+			const float v1 = s[e0[e]], v2 = s[e1[e]];
+			if ((v1 >= 0.0f) == (v2 >= 0.0f)) continue;
+			const float p1x = fi + float(cx[e0[e]]), p1y = fj + float(cy[e0[e]]), p1z = fk + float(cz[e0[e]]);
+			const float p2x = fi + float(cx[e1[e]]), p2y = fj + float(cy[e1[e]]), p2z = fk + float(cz[e1[e]]);
+			const float tt = (0.0f - v1) / (v2 - v1);
+			sx_ = sx_ + (p1x + tt * (p2x - p1x));
+			sy_ = sy_ + (p1y + tt * (p2y - p1y));
+			sz_ = sz_ + (p1z + tt * (p2z - p1z));
+			++n;
+		}
+		const float fn = float(n);
+		const float gcx = sx_ / fn, gcy = sy_ / fn, gcz = sz_ / fn;
+		float* v = a.vertices + size_t(id) * 3;
+		v[0] = a.gx + (maxx - a.gx) * (gcx - 0.0f) / (fsx - 0.0f);
+		v[1] = a.gy + (maxy - a.gy) * (gcy - 0.0f) / (fsy - 0.0f);
+		v[2] = a.gz + (maxz - a.gz) * (gcz - 0.0f) / (fsz - 0.0f);
+		++id;
+	}
+}
+
+// Quads of one active cube: writes up to 6 triangles when `out` is set and
+// returns the quad count.
+__device__ __forceinline__ uint32_t cube_quads(const MeshArgs& a, uint32_t i, uint32_t j, uint32_t k, uint32_t* out)
+{
+	if (i == 0 || j == 0 || k == 0) return 0;
+	// neighbour cubes 0..5 (surface_nets.cpp:415-423)
+	const uint32_t nx[6] = { i - 1, i - 1, i, i, i, i - 1 };
+	const uint32_t ny[6] = { j, j - 1, j - 1, j - 1, j, j };
+	const uint32_t nz[6] = { k, k, k, k - 1, k - 1, k - 1 };
+	constexpr uint8_t quad[3][3] = { { 0, 1, 2 }, { 0, 5, 4 }, { 2, 3, 4 } };
+	uint32_t count = 0;
+	for (int q = 0; q < 3; ++q)
+	{
+		bool all = true;
+#pragma unroll
+		for (int t = 0; t < 3; ++t) all = all && cube_active(a, nx[quad[q][t]], ny[quad[q][t]], nz[quad[q][t]]);
+		if (!all) continue;
+		if (out)
+		{
+			float first, second;
+			if (q == 0) { first = pool_at(a, i, j, k); second = pool_at(a, i, j, k + 1); }
+			else if (q == 1) { first = pool_at(a, i, j + 1, k); second = pool_at(a, i, j, k); }
+			else { first = pool_at(a, i, j, k); second = pool_at(a, i + 1, j, k); }
+			uint32_t n[3];
+#pragma unroll
+			for (int t = 0; t < 3; ++t) n[t] = vertex_id(a, nx[quad[q][t]], ny[quad[q][t]], nz[quad[q][t]]);
+			const bool fwd = second > first;
+			const uint32_t v0 = vertex_id(a, i, j, k);
+			const uint32_t v1 = fwd ? n[0] : n[2], v2 = n[1], v3 = fwd ? n[2] : n[0];
+			uint32_t* tri = out + size_t(count) * 6;
+			tri[0] = v0; tri[1] = v1; tri[2] = v2;
+			tri[3] = v0; tri[4] = v2; tri[5] = v3;
+		}
+		++count;
+	}
+	return count;
+}
+
+__global__ void __launch_bounds__(256) k_quad_count(MeshArgs a)
+{
+	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	const uint64_t rows = uint64_t(a.sy) * a.sz;
+	if (t >= rows * a.aw) return;
+	uint64_t act = a.active[t];
+	uint32_t total = 0;
+	if (act)
+	{
+		const uint32_t w = uint32_t(t % a.aw);
+		const uint64_t row = t / a.aw;
+		const uint32_t j = uint32_t(row % a.sy), k = uint32_t(row / a.sy);
+		while (act)
+		{
+			const int b = __ffsll((long long)act) - 1;
+			act &= act - 1;
+			total += cube_quads(a, w * 64 + uint32_t(b), j, k, nullptr);
+		}
+	}
+	a.quad_count[t] = total;
+}
+
+__global__ void __launch_bounds__(256) k_quad_write(MeshArgs a)
+{
+	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	const uint64_t rows = uint64_t(a.sy) * a.sz;
+	if (t >= rows * a.aw) return;
+	uint64_t act = a.active[t];
+	if (!act) return;
+	const uint32_t w = uint32_t(t % a.aw);
+	const uint64_t row = t / a.aw;
+	const uint32_t j = uint32_t(row % a.sy), k = uint32_t(row / a.sy);
+	uint32_t* out = a.triangles + size_t(a.quad_base[t]) * 6;
+	while (act)
+	{
+		const int b = __ffsll((long long)act) - 1;
+		act &= act - 1;
+		out += size_t(cube_quads(a, w * 64 + uint32_t(b), j, k, out)) * 6;
+	}
+}
+
+// Pool tiles -> dense x-fastest grid (for copying samples back to the host).
+__global__ void __launch_bounds__(256) k_pool_to_dense(MeshArgs a, float* dense, uint32_t k0, uint32_t nk)
+{
+	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	const uint64_t plane = uint64_t(a.nx) * a.ny;
+	if (t >= plane * nk) return;
+	const uint32_t i = uint32_t(t % a.nx);
+	const uint32_t j = uint32_t((t / a.nx) % a.ny);
+	const uint32_t k = k0 + uint32_t(t / plane);
+	dense[t] = pool_at(a, i, j, k);
+}
+
+// Dense x-fastest grid -> pool tiles and sign rows (host-supplied samples).
+__global__ void __launch_bounds__(256) k_dense_to_pool(MeshArgs a, const float* dense, float* pool)
+{
+	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	const uint64_t plane = uint64_t(a.nx) * a.ny;
+	if (t >= plane * a.nz) return;
+	const uint32_t i = uint32_t(t % a.nx);
+	const uint32_t j = uint32_t((t / a.nx) % a.ny);
+	const uint32_t k = uint32_t(t / plane);
+	const uint32_t tile = (i >> 3) + a.tx * ((j >> 3) + a.ty * (k >> 3));
+	const float v = dense[t];
+	pool[size_t(tile) * kTileSamples + ((k & 7) * 64 + (j & 7) * 8 + (i & 7))] = v;
+}
+
+__global__ void __launch_bounds__(256) k_dense_signs(MeshArgs a, const float* dense, uint8_t* signs)
+{
+	// Thread per sign byte.
+	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	const uint64_t rows = uint64_t(a.ny) * a.nz;
+	if (t >= rows * a.row_bytes) return;
+	const uint32_t b = uint32_t(t % a.row_bytes);
+	const uint64_t row = t / a.row_bytes;
+	uint8_t byte = 0;
+	for (int x = 0; x < 8; ++x)
+	{
+		const uint32_t i = b * 8 + x;
+		if (i < a.nx && dense[row * a.nx + i] >= 0.0f) byte |= uint8_t(1u << x);
+	}
+	signs[t] = byte;
+}
+
+inline unsigned blocks_for(uint64_t n, unsigned threads) { return unsigned((n + threads - 1) / threads); }
+
+}  // namespace
+
+hipError_t mesh_classify(const MeshArgs& a, hipStream_t s)
+{
+	const uint64_t n = uint64_t(a.sy) * a.sz * a.aw;
+	hipLaunchKernelGGL(k_classify, dim3(blocks_for(n, 256)), dim3(256), 0, s, a);
+	return hipGetLastError();
+}
+
+hipError_t mesh_vertices(const MeshArgs& a, hipStream_t s)
+{
+	const uint64_t n = uint64_t(a.sy) * a.sz * a.aw;
+	hipLaunchKernelGGL(k_vertices, dim3(blocks_for(n, 256)), dim3(256), 0, s, a);
+	return hipGetLastError();
+}
+
+hipError_t mesh_quad_count(const MeshArgs& a, hipStream_t s)
+{
+	const uint64_t n = uint64_t(a.sy) * a.sz * a.aw;
+	hipLaunchKernelGGL(k_quad_count, dim3(blocks_for(n, 256)), dim3(256), 0, s, a);
+	return hipGetLastError();
+}
+
+hipError_t mesh_quad_write(const MeshArgs& a, hipStream_t s)
+{
+	const uint64_t n = uint64_t(a.sy) * a.sz * a.aw;
+	hipLaunchKernelGGL(k_quad_write, dim3(blocks_for(n, 256)), dim3(256), 0, s, a);
+	return hipGetLastError();
+}
+
+hipError_t mesh_pool_to_dense(const MeshArgs& a, float* dense, uint32_t k0, uint32_t nk, hipStream_t s)
+{
+	const uint64_t n = uint64_t(a.nx) * a.ny * nk;
+	hipLaunchKernelGGL(k_pool_to_dense, dim3(blocks_for(n, 256)), dim3(256), 0, s, a, dense, k0, nk);
+	return hipGetLastError();
+}
+
+hipError_t mesh_dense_to_pool(const MeshArgs& a, const float* dense, hipStream_t s)
+{
+	const uint64_t n = uint64_t(a.nx) * a.ny * a.nz;
+	hipLaunchKernelGGL(k_dense_to_pool, dim3(blocks_for(n, 256)), dim3(256), 0, s, a, dense, const_cast<float*>(a.pool));
+	const uint64_t m = uint64_t(a.ny) * a.nz * a.row_bytes;
+	hipLaunchKernelGGL(k_dense_signs, dim3(blocks_for(m, 256)), dim3(256), 0, s, a, dense, const_cast<uint8_t*>(a.signs));
+	return hipGetLastError();
+}
+
+hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, void* temp, size_t& temp_bytes, hipStream_t s)
+{
+	return hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, in, out, int(n), s);
+}
+
+}  // namespace tg

@@ -1,0 +1,31 @@
+// Reference ProgramBuffer words -> device instruction records.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "tg_types.h"
+
+namespace tg {
+
+struct DecodedProgram
+{
+	std::vector<Instr> instrs;   // ends with OP_STOP
+	uint32_t max_stack = 0;      // interpreter stack high-water mark
+	uint32_t brushes = 0;
+	uint32_t set_ops = 0;
+	bool lipschitz = true;       // false if any ellipsoid (not a distance bound)
+	double flops = 0.0;          // algorithmic FP32 work per evaluation (cost table)
+};
+
+// Validates stack discipline and opcodes the way SDFInterpreter::Eval would
+// (tangerine/sdf_evaluator.cpp:1397-1600) and re-encodes the program.
+// Returns false with a message on malformed input.
+bool decode_words(const uint32_t* words, size_t count, DecodedProgram& out, std::string& error);
+
+// Algorithmic cost of one evaluation of a record (adds, muls, min/max and
+// selects count 1, an FMA 2, sqrt and divide 1 each).  Fixed table.
+double instr_flops(const Instr& in);
+
+}  // namespace tg
