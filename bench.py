@@ -1,0 +1,221 @@
+"""Benchmark: grid export of a seeded random CSG tree on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--workload export1024|export512] [--no-cpu]
+
+A step is one plain exact export of the workload: every sample of the grid
+interpreted on the GPU, then naive surface nets on the device.  The tree is
+built and uploaded before timing; the grid, the mesh and all intermediates
+stay in HBM.  One JSON line is printed (see the keys below).
+
+Workloads (named by what they are):
+  export1024  2048 random primitives (seed 1), 1024^3 cells over [-8,8]^3
+              (1025^3 = 1.08e9 samples)
+  export512   256 random primitives (seed 1), 512^3 cells over [-6,6]^3
+
+cpu_baseline: the reference's own threaded export (its SDFOctree at 0.25
+cells behind the surface-nets vertex and face thunks, built from the
+reference sources into oracle/_ref) on this host, over a bounded number of
+z-slabs; the full-grid time is extrapolated from them.  Without oracle/_ref
+the oracle's interpreter fills the same slabs ("port").
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+from tangerine_amd import _lib, scenes  # noqa: E402
+
+PEAK_FP32_TFLOPS = 157.3          # MI355X published FP32 vector peak (packed issue)
+PEAK_FP32_NONPACKED_TFLOPS = 78.6  # same without packed FP32 (not used by these kernels)
+PEAK_HBM_GBS = 8000.0
+
+WORKLOADS = {
+    "export1024": dict(seed=1, primitives=2048, lo=-8.0, hi=8.0, cells=1024),
+    "export512": dict(seed=1, primitives=256, lo=-6.0, hi=6.0, cells=512),
+}
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+class Mesher:
+    def __init__(self, lib):
+        self.lib = lib
+        h = C.c_void_p()
+        _lib.check(lib.tg_mesher_create(C.byref(h)), "tg_mesher_create")
+        self.h = h.value
+
+    def export(self, prog, grid, flags):
+        _lib.check(self.lib.tg_mesher_export(self.h, prog.handle, C.byref(grid), flags), "tg_mesher_export")
+        st = _lib.TgExportStats()
+        _lib.check(self.lib.tg_mesher_stats(self.h, C.byref(st)), "tg_mesher_stats")
+        return st
+
+    def counts(self):
+        nv, nt = C.c_uint64(), C.c_uint64()
+        _lib.check(self.lib.tg_mesher_counts(self.h, C.byref(nv), C.byref(nt)), "counts")
+        return nv.value, nt.value
+
+    def close(self):
+        self.lib.tg_mesher_destroy(self.h)
+
+
+def cpu_baseline(spec, lo, hi, cells, budget_s: float, threads: int) -> dict:
+    """Reference threaded export on `threads` host threads over a few z-slabs."""
+    origin, step, cc = scenes.grid_for(lo, hi, cells)
+    try:
+        from oracle.oracle import REF_SO, Oracle, Reference
+    except Exception as exc:  # pragma: no cover
+        return {"error": f"oracle unavailable: {exc}"}
+    if REF_SO.exists():
+        ref = Reference()
+        h = ref.build(spec)
+        t0 = time.perf_counter()
+        octree = ref.octree(h, 0.25)
+        t_oct = time.perf_counter() - t0
+        k0 = cells // 2
+        slabs, elapsed, evals = 0, 0.0, 0
+        while elapsed < budget_s and k0 + slabs < cells:
+            t0 = time.perf_counter()
+            _, _, ev = ref.export_threaded(octree, origin, step, cc, k0 + slabs, k0 + slabs + 1, threads)
+            elapsed += time.perf_counter() - t0
+            evals += ev
+            slabs += 1
+        ref.lib.tgref_octree_free(octree)
+        ref.free(h)
+        rate = evals / elapsed
+        return {
+            "value": rate, "unit": "evals/s", "cores": threads, "kind": "reference",
+            "sample": (f"reference threaded export (SDFOctree 0.25 + surface-nets thunks) over {slabs} of "
+                       f"{cells} z-slabs at the grid centre, {evals} SDFOctree::Eval calls in {elapsed:.1f} s; "
+                       f"octree build {t_oct:.1f} s"),
+            "export_ms_extrapolated": 1000.0 * (t_oct + elapsed * cells / slabs),
+            "hardware_concurrency": int(ref.lib.tgref_hardware_concurrency()),
+        }
+    orc = Oracle()
+    h = orc.build(spec)
+    words = orc.compile(h)
+    k0 = cells // 2
+    planes, elapsed = 0, 0.0
+    while elapsed < budget_s and planes < cells:
+        t0 = time.perf_counter()
+        orc.grid_interp(words, origin, step, cc, k0 + planes, k0 + planes + 1, threads)
+        elapsed += time.perf_counter() - t0
+        planes += 1
+    evals = planes * (cells + 1) ** 2
+    return {
+        "value": evals / elapsed, "unit": "evals/s", "cores": threads, "kind": "port",
+        "sample": f"oracle interpreter over {planes} z-planes of the grid, {evals} evaluations in {elapsed:.1f} s",
+        "export_ms_extrapolated": None,
+    }
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="export1024", choices=sorted(WORKLOADS))
+    ap.add_argument("--fast", action="store_true", help="contracted FP32 numerics instead of bit parity")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("TG_CPU_THREADS", "16")))
+    args = ap.parse_args(argv)
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; run with --gpus 1 (multi-GPU sharding is "
+                                   "described in DESIGN.md but not built)"}))
+        return 2
+    if int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(json.dumps({"error": "single-GPU benchmark; do not launch under torch.distributed"}))
+        return 2
+
+    wl = WORKLOADS[args.workload]
+    lib = _lib.load()
+    _lib.check(lib.tg_device_init(0), "tg_device_init")
+    spec = scenes.random_csg_spec(wl["seed"], wl["primitives"])
+    node = scenes.build_sdf(spec)
+    prog = node.program()
+    origin, step, cells = scenes.grid_for(wl["lo"], wl["hi"], wl["cells"])
+    grid = _lib.grid_struct(origin, step, cells)
+    flags = _lib.TG_FLAG_FAST if args.fast else 0
+    mesher = Mesher(lib)
+    log(f"workload {args.workload}: {prog.info}")
+
+    for i in range(args.warmup):
+        st = mesher.export(prog, grid, flags)
+        log(f"warmup {i}: eval {st.eval_ms:.1f} ms mesh {st.mesh_ms:.1f} ms")
+    _lib.check(lib.tg_device_synchronize(), "sync")
+    evals_ms, mesh_ms, totals = [], [], []
+    flops = evals = 0
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        st = mesher.export(prog, grid, flags)
+        evals_ms.append(st.eval_ms)
+        mesh_ms.append(st.mesh_ms)
+        totals.append(st.total_ms)
+        flops, evals = st.flops, st.exact_evals
+        log(f"step {i}: eval {st.eval_ms:.1f} ms mesh {st.mesh_ms:.1f} ms total {st.total_ms:.1f} ms")
+    _lib.check(lib.tg_device_synchronize(), "sync")
+    wall = time.perf_counter() - t0
+    nverts, ntris = mesher.counts()
+    mesher.close()
+
+    ms_per_step = 1000.0 * wall / args.steps
+    samples = (wl["cells"] + 1) ** 3
+    eval_ms = float(np.mean(evals_ms))
+    achieved = flops / (eval_ms * 1e-3) / 1e12
+    out = {
+        "metric": "exact SDF evals/s vs CPU; 1024^3 export ms, plain vs surrogate-culled, same mesh",
+        "value": evals / (ms_per_step * 1e-3),
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded random CSG tree, generated in-process)",
+        "config": {
+            "workload": f"{args.workload}: plain exact export (grid eval + naive surface nets)",
+            "primitives": wl["primitives"], "seed": wl["seed"], "cells": wl["cells"],
+            "bounds": [wl["lo"], wl["hi"]], "samples": samples, "numerics": "fast" if args.fast else "exact",
+        },
+        "export_ms_plain": float(np.mean(totals)),
+        "eval_ms": eval_ms,
+        "mesh_ms": float(np.mean(mesh_ms)),
+        "samples_per_s": samples / (ms_per_step * 1e-3),
+        "mesh": {"vertices": nverts, "triangles": ntris},
+        "roofline": {
+            "bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+            "frac_of_nonpacked_peak": achieved / PEAK_FP32_NONPACKED_TFLOPS,
+            "kernel": "k_eval_tiles", "flops_per_launch": flops, "launch_ms": eval_ms,
+        },
+    }
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(spec, wl["lo"], wl["hi"], wl["cells"], args.cpu_seconds, args.cpu_threads)
+        cb = out["cpu_baseline"]
+        if cb.get("value"):
+            out["gpu_over_cpu_evals"] = out["value"] / cb["value"]
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

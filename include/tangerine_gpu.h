@@ -93,6 +93,10 @@ typedef struct tg_export_stats
 const char* tg_last_error(void);
 int tg_device_init(int device);
 int tg_device_synchronize(void);
+/* Measurement utility (no reference counterpart): copies `bytes` between two
+ * device allocations it makes itself with a plain streaming kernel, `repeat`
+ * times; used to calibrate HBM byte counters against a known traffic. */
+int tg_device_copy_probe(uint64_t bytes, int repeat, double* ms_per_copy);
 
 /* ---- tree builders (SDF::* factories; radii and half extents) -------------- */
 tg_node* tg_sphere(float radius);                                        /* sdf_evaluator.cpp:1206 */
@@ -132,6 +136,9 @@ int tg_node_compile(const tg_node* node, uint32_t* words, uint64_t capacity, uin
 int tg_program_from_node(const tg_node* node, tg_program** out);
 int tg_program_from_words(const uint32_t* words, uint64_t count, tg_program** out);
 int tg_program_info_get(const tg_program* program, tg_program_info* out);
+/* Host only: validates words as tg_program_from_words does and reports the
+ * program's properties without touching a device. */
+int tg_words_inspect(const uint32_t* words, uint64_t count, tg_program_info* out);
 void tg_program_destroy(tg_program* program);
 
 /* ---- point queries (EvalTree / SDFNode::Eval, batched) --------------------- */

@@ -46,6 +46,7 @@ SIGNATURES = {
     "tg_last_error": (C.c_char_p, []),
     "tg_device_init": (C.c_int, [C.c_int]),
     "tg_device_synchronize": (C.c_int, []),
+    "tg_device_copy_probe": (C.c_int, [_u64, C.c_int, C.POINTER(C.c_double)]),
     "tg_sphere": (_p, [_f]),
     "tg_ellipsoid": (_p, [_f, _f, _f]),
     "tg_box": (_p, [_f, _f, _f]),
@@ -77,6 +78,7 @@ SIGNATURES = {
     "tg_program_from_words": (C.c_int, [_u32p, _u64, C.POINTER(_p)]),
     "tg_program_info_get": (C.c_int, [_p, C.POINTER(TgProgramInfo)]),
     "tg_program_destroy": (None, [_p]),
+    "tg_words_inspect": (C.c_int, [_u32p, _u64, C.POINTER(TgProgramInfo)]),
     "tg_eval_points": (C.c_int, [_p, _fp, _u64, _fp, _u32]),
     "tg_eval_points_device": (C.c_int, [_p, _p, _u64, _p, _u32, _p]),
     "tg_mesher_create": (C.c_int, [C.POINTER(_p)]),

@@ -106,6 +106,48 @@ extern "C" int tg_device_synchronize(void)
 	return TG_OK;
 }
 
+// Plain streaming copy, 16 bytes per lane per iteration.
+__global__ void __launch_bounds__(256) k_copy_probe(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n)
+{
+	const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+	for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+extern "C" int tg_device_copy_probe(uint64_t bytes, int repeat, double* ms_per_copy)
+{
+	int rc = ensure_device();
+	if (rc) return rc;
+	if (bytes < 16 || repeat < 1) return fail(TG_ERR_INVALID, "tg_device_copy_probe: bytes >= 16, repeat >= 1");
+	const uint64_t n = bytes / 16;
+	DevBuf a, b;
+	rc = a.ensure(n * 16);
+	if (!rc) rc = b.ensure(n * 16);
+	if (rc)
+	{
+		a.release();
+		return rc;
+	}
+	hipEvent_t e0, e1;
+	hipEventCreate(&e0);
+	hipEventCreate(&e1);
+	hipMemset(a.ptr, 1, n * 16);
+	const unsigned blocks = unsigned(std::min<uint64_t>((n + 255) / 256, 256u * 8u * 8u));
+	hipEventRecord(e0, nullptr);
+	for (int r = 0; r < repeat; ++r)
+		hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, nullptr, a.as<const uint4>(), b.as<uint4>(), n);
+	hipEventRecord(e1, nullptr);
+	hipError_t e = hipEventSynchronize(e1);
+	float ms = 0;
+	hipEventElapsedTime(&ms, e0, e1);
+	hipEventDestroy(e0);
+	hipEventDestroy(e1);
+	a.release();
+	b.release();
+	if (e != hipSuccess) return hip_fail(e, "copy probe");
+	if (ms_per_copy) *ms_per_copy = double(ms) / repeat;
+	return TG_OK;
+}
+
 // ---------------------------------------------------------------------------
 // builders
 
@@ -260,9 +302,16 @@ int tg::upload_program(const DecodedProgram& d, tg_program** out)
 	auto p = std::make_unique<tg_program>();
 	p->decoded = d;
 	rc = p->device.ensure(d.instrs.size() * sizeof(Instr));
-	if (rc) return rc;
+	if (!rc) rc = p->entries.ensure(std::max<size_t>(d.entries.size(), 1) * sizeof(HostEntry));
+	if (rc)
+	{
+		p->device.release();
+		return rc;
+	}
 	TG_CHECK_HIP(hipMemcpy(p->device.ptr, d.instrs.data(), d.instrs.size() * sizeof(Instr), hipMemcpyHostToDevice),
 		"program upload");
+	TG_CHECK_HIP(hipMemcpy(p->entries.ptr, d.entries.data(), d.entries.size() * sizeof(HostEntry), hipMemcpyHostToDevice),
+		"program entries upload");
 	*out = p.release();
 	return TG_OK;
 }
@@ -288,15 +337,31 @@ extern "C" int tg_program_from_node(const tg_node* n, tg_program** out)
 	return tg_program_from_words(w.data(), w.size(), out);
 }
 
+static void fill_info(const DecodedProgram& d, tg_program_info* out)
+{
+	out->instructions = d.instrs.size();
+	out->max_stack = d.max_stack;
+	out->brushes = d.brushes;
+	out->set_ops = d.set_ops;
+	out->lipschitz = d.lipschitz ? 1u : 0u;
+	out->flops_per_eval = d.flops;
+}
+
 extern "C" int tg_program_info_get(const tg_program* p, tg_program_info* out)
 {
 	if (!p || !out) return fail(TG_ERR_INVALID, "tg_program_info_get: null argument");
-	out->instructions = p->decoded.instrs.size();
-	out->max_stack = p->decoded.max_stack;
-	out->brushes = p->decoded.brushes;
-	out->set_ops = p->decoded.set_ops;
-	out->lipschitz = p->decoded.lipschitz ? 1u : 0u;
-	out->flops_per_eval = p->decoded.flops;
+	fill_info(p->decoded, out);
+	return TG_OK;
+}
+
+extern "C" int tg_words_inspect(const uint32_t* words, uint64_t count, tg_program_info* out)
+{
+	if (!words || !out || count == 0) return fail(TG_ERR_INVALID, "tg_words_inspect: null or empty argument");
+	DecodedProgram d;
+	std::string err;
+	if (!decode_words(words, count, d, err))
+		return fail(err.find("stack slots") != std::string::npos ? TG_ERR_LIMIT : TG_ERR_PROGRAM, "tg_words_inspect: " + err);
+	fill_info(d, out);
 	return TG_OK;
 }
 
@@ -304,6 +369,7 @@ extern "C" void tg_program_destroy(tg_program* p)
 {
 	if (!p) return;
 	p->device.release();
+	p->entries.release();
 	delete p;
 }
 

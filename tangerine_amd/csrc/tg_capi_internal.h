@@ -30,7 +30,8 @@ struct DevBuf
 struct tg_program
 {
 	tg::DecodedProgram decoded;
-	tg::DevBuf device;  // Instr records
+	tg::DevBuf device;   // Instr records
+	tg::DevBuf entries;  // whole program as pruning entries
 };
 
 namespace tg {

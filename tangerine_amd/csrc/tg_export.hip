@@ -29,11 +29,14 @@ struct tg_mesher
 	hipStream_t stream = nullptr;
 	hipEvent_t ev[4] = {};
 	DevBuf pool, signs, active, word_count, word_base, quad_count, quad_base, vertices, triangles, scan_tmp, dense;
+	DevBuf block_pool, block_desc, counters;
 	MeshArgs args{};
 	uint64_t nverts = 0, ntris = 0;
 	bool have_grid = false;
 	tg_export_stats stats{};
-	int points_per_lane = 8;
+	int points_per_lane = 8;      // preferred points per lane (tuned)
+	size_t lds_budget = 80 * 1024;  // LDS per workgroup the block kernel may use
+	int last_points_per_lane = 0;
 };
 
 namespace {
@@ -127,6 +130,92 @@ int surface_nets(tg_mesher* m)
 	return TG_OK;
 }
 
+constexpr size_t kMaxBlockPoolBytes = size_t(8) << 30;
+constexpr size_t kMaxLds = 160 * 1024;
+
+// Two-level pruned evaluation of every tile (see k_prune_blocks / k_eval_blocks).
+// Returns 1 if it ran, 0 if the program or grid is outside its limits (the
+// caller then interprets the whole program per tile), <0 on error.
+int eval_pruned(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t flags, const uint64_t* tile_mask,
+	const uint32_t* block_list, uint32_t n_listed, uint64_t* evals, double* flops)
+{
+	const DecodedProgram& d = p->decoded;
+	MeshArgs& ma = m->args;
+	if (!d.lipschitz || d.entries.empty() || d.entries.size() >= 65535) return 0;
+	const uint32_t bx = (ma.tx + kBlockTiles - 1) / kBlockTiles;
+	const uint32_t by = (ma.ty + kBlockTiles - 1) / kBlockTiles;
+	const uint32_t bz = (ma.tz + kBlockTiles - 1) / kBlockTiles;
+	const uint64_t nblocks = uint64_t(bx) * by * bz;
+	const uint32_t cap = uint32_t(d.entries.size());
+	const uint32_t whole_slots = std::max<uint32_t>(1, d.max_stack - 1);
+	if (nblocks * cap * sizeof(Entry) > kMaxBlockPoolBytes) return 0;
+	if (prune_lds_bytes(d.binary_entries, whole_slots) > 64 * 1024) return 0;
+	int rc = m->block_pool.ensure(nblocks * cap * sizeof(Entry));
+	if (!rc) rc = m->block_desc.ensure(nblocks * sizeof(BlockDesc));
+	if (!rc) rc = m->counters.ensure(64);
+	if (rc) return rc;
+	uint32_t* max_stack = m->counters.as<uint32_t>();
+	unsigned long long* work = reinterpret_cast<unsigned long long*>(m->counters.as<unsigned char>() + 16);
+	TG_CHECK_HIP(hipMemsetAsync(m->counters.ptr, 0, 64, m->stream), "counter reset");
+
+	PruneArgs pa{};
+	pa.prog = static_cast<const Instr*>(p->device.ptr);
+	pa.entries = p->entries.as<const Entry>();
+	pa.n_entries = cap;
+	pa.n_binary = d.binary_entries;
+	pa.gx = g->x; pa.gy = g->y; pa.gz = g->z;
+	pa.dx = g->dx; pa.dy = g->dy; pa.dz = g->dz;
+	pa.bx = bx; pa.by = by; pa.bz = bz;
+	pa.slots = whole_slots;
+	pa.whole_flops = uint32_t(d.flops);
+	pa.pool = m->block_pool.as<Entry>();
+	pa.cap = cap;
+	pa.desc = m->block_desc.as<BlockDesc>();
+	pa.max_stack = max_stack;
+	hipError_t e = (flags & TG_FLAG_FAST) ? fast::launch_prune_blocks(pa, m->stream)
+	                                      : exact::launch_prune_blocks(pa, m->stream);
+	if (e != hipSuccess) return hip_fail(e, "block pruning launch");
+	uint32_t block_stack = 0;
+	TG_CHECK_HIP(hipMemcpyAsync(&block_stack, max_stack, 4, hipMemcpyDeviceToHost, m->stream), "stack read");
+	TG_CHECK_HIP(hipStreamSynchronize(m->stream), "block pruning");
+
+	BlockEvalArgs ba{};
+	ba.prog = pa.prog;
+	ba.pool = pa.pool;
+	ba.desc = pa.desc;
+	ba.block_list = block_list;
+	ba.tile_mask = tile_mask;
+	ba.nblocks = block_list ? n_listed : uint32_t(nblocks);
+	ba.gx = g->x; ba.gy = g->y; ba.gz = g->z;
+	ba.dx = g->dx; ba.dy = g->dy; ba.dz = g->dz;
+	ba.nx = ma.nx; ba.ny = ma.ny; ba.nz = ma.nz;
+	ba.tx = ma.tx; ba.ty = ma.ty; ba.tz = ma.tz;
+	ba.bx = bx; ba.by = by; ba.bz = bz;
+	ba.tile_slot = ma.tile_slot;
+	ba.out_pool = m->pool.as<float>();
+	ba.signs = m->signs.as<uint8_t>();
+	ba.row_bytes = ma.row_bytes;
+	ba.slots = std::max<uint32_t>(1, block_stack > 0 ? block_stack - 1 : 1);
+	ba.prune_tiles = 1;
+	ba.work = work;
+	ba.whole = pa.entries;
+	ba.n_whole = cap;
+	ba.waves = 4;
+	int pts = m->points_per_lane;
+	while (pts > 2 && block_lds_bytes(ba.slots, pts, ba.waves) > m->lds_budget) pts >>= 1;
+	if (block_lds_bytes(ba.slots, pts, ba.waves) > kMaxLds) return 0;
+	m->last_points_per_lane = pts;
+	e = (flags & TG_FLAG_FAST) ? fast::launch_eval_blocks(ba, pts, m->stream)
+	                           : exact::launch_eval_blocks(ba, pts, m->stream);
+	if (e != hipSuccess) return hip_fail(e, "block evaluation launch");
+	unsigned long long wk[2] = { 0, 0 };
+	TG_CHECK_HIP(hipMemcpyAsync(wk, work, 16, hipMemcpyDeviceToHost, m->stream), "work read");
+	TG_CHECK_HIP(hipStreamSynchronize(m->stream), "block evaluation");
+	*evals = wk[0];
+	*flops = double(wk[1]);
+	return 1;
+}
+
 }  // namespace
 
 extern "C" int tg_mesher_create(tg_mesher** out)
@@ -151,7 +240,8 @@ extern "C" void tg_mesher_destroy(tg_mesher* m)
 	if (!m) return;
 	if (m->stream) hipStreamSynchronize(m->stream);
 	for (DevBuf* b : { &m->pool, &m->signs, &m->active, &m->word_count, &m->word_base, &m->quad_count, &m->quad_base,
-	                   &m->vertices, &m->triangles, &m->scan_tmp, &m->dense })
+	                   &m->vertices, &m->triangles, &m->scan_tmp, &m->dense, &m->block_pool, &m->block_desc,
+	                   &m->counters })
 		b->release();
 	for (hipEvent_t e : m->ev)
 		if (e) hipEventDestroy(e);
@@ -183,9 +273,22 @@ extern "C" int tg_mesher_export(tg_mesher* m, const tg_program* p, const tg_grid
 	ga.slots = std::max<uint32_t>(1, p->decoded.max_stack - 1);
 
 	TG_CHECK_HIP(hipEventRecord(m->ev[0], m->stream), "event");
-	hipError_t e = (flags & TG_FLAG_FAST) ? fast::launch_tiles(ga, m->points_per_lane, m->stream)
-	                                      : exact::launch_tiles(ga, m->points_per_lane, m->stream);
-	if (e != hipSuccess) return hip_fail(e, "tile evaluation launch");
+	uint64_t evals = uint64_t(ga.ntiles) * kTileSamples;
+	double flops = double(evals) * p->decoded.flops;
+	int pruned = 0;
+	if (!(flags & TG_FLAG_NO_PRUNE))
+	{
+		pruned = eval_pruned(m, p, g, flags, nullptr, nullptr, 0, &evals, &flops);
+		if (pruned < 0) return pruned;
+	}
+	if (!pruned)
+	{
+		evals = uint64_t(ga.ntiles) * kTileSamples;
+		flops = double(evals) * p->decoded.flops;
+		hipError_t e = (flags & TG_FLAG_FAST) ? fast::launch_tiles(ga, 8, m->stream)
+		                                      : exact::launch_tiles(ga, 8, m->stream);
+		if (e != hipSuccess) return hip_fail(e, "tile evaluation launch");
+	}
 	TG_CHECK_HIP(hipEventRecord(m->ev[1], m->stream), "event");
 	rc = surface_nets(m);
 	if (rc) return rc;
@@ -201,9 +304,9 @@ extern "C" int tg_mesher_export(tg_mesher* m, const tg_program* p, const tg_grid
 	s.mesh_ms = t_mesh;
 	s.total_ms = t_all;
 	s.samples = uint64_t(ma.nx) * ma.ny * ma.nz;
-	s.tiles_evaluated = ga.ntiles;
-	s.exact_evals = uint64_t(ga.ntiles) * kTileSamples;
-	s.flops = double(s.exact_evals) * p->decoded.flops;
+	s.tiles_evaluated = evals / kTileSamples;
+	s.exact_evals = evals;
+	s.flops = flops;
 	m->have_grid = true;
 	return TG_OK;
 }

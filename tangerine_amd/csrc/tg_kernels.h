@@ -8,6 +8,12 @@
 
 namespace tg {
 
+// Dynamic LDS bytes per wave for the interpreter spill region.
+__host__ __device__ inline size_t spill_bytes_per_wave(uint32_t slots, int points_per_lane)
+{
+	return size_t(slots) * size_t(points_per_lane) * 64 * sizeof(float);
+}
+
 struct PointArgs
 {
 	const Instr* prog;
@@ -33,21 +39,102 @@ struct GridArgs
 	uint32_t slots;
 };
 
+// ---- pruned programs ----------------------------------------------------------
+//
+// A program is also viewed as a list of entries in postfix order, each naming
+// a record and its children, so that it can be pruned per cell without
+// touching the records: entry i of a pruned program says "execute record
+// `rec`" (or negate the top of stack), and where its operands' roots are.
+struct Entry
+{
+	uint32_t rec;   // record index (bits 0..23) | opcode << 24 (OP_NEGATE: no record)
+	uint32_t l;     // binary: position of the left operand's root; unary: operand root
+	uint32_t r;     // binary: position of the right operand's root
+	uint32_t ord;   // binary: ordinal among the program's binary entries
+};
+
+constexpr uint32_t kRecMask = 0x00ffffffu;
+constexpr int kBlockTiles = 4;                 // tiles per block edge
+constexpr int kTilesPerBlock = 64;             // 4^3 tiles = one lane each
+constexpr int kBlockSamples = kTile * kBlockTiles;
+
+struct BlockDesc
+{
+	uint32_t offset;     // first entry in the block-program pool
+	uint32_t length;     // entries
+	uint32_t max_stack;  // interpreter stack high-water mark
+	uint32_t flops;      // cost-table work of one evaluation
+};
+
+// Level 1: prune the whole program at every block centre.
+struct PruneArgs
+{
+	const Instr* prog;
+	const Entry* entries;   // whole program as entries
+	uint32_t n_entries;
+	uint32_t n_binary;
+	float gx, gy, gz, dx, dy, dz;
+	uint32_t bx, by, bz;    // blocks per axis
+	uint32_t slots;         // LDS spill slots (whole-program max stack - 1)
+	uint32_t whole_flops;   // cost of one whole-program evaluation
+	Entry* pool;            // block programs, `cap` entries per block
+	uint32_t cap;
+	BlockDesc* desc;
+	uint32_t* max_stack;    // atomic max over blocks
+};
+
+// Level 2 + evaluation: one workgroup per block.
+struct BlockEvalArgs
+{
+	const Instr* prog;
+	const Entry* pool;
+	const BlockDesc* desc;
+	const uint32_t* block_list;  // blocks to run (null: all)
+	const uint64_t* tile_mask;   // per block: tiles to evaluate (null: all)
+	uint32_t nblocks;            // launched blocks
+	float gx, gy, gz, dx, dy, dz;
+	uint32_t nx, ny, nz;
+	uint32_t tx, ty, tz;
+	uint32_t bx, by, bz;
+	const uint32_t* tile_slot;
+	float* out_pool;
+	uint8_t* signs;
+	uint32_t row_bytes;
+	uint32_t slots;              // LDS spill slots (max block stack - 1)
+	int prune_tiles;             // 0: evaluate block programs unpruned per tile
+	unsigned long long* work;    // [0] samples interpreted, [1] flops units executed
+	const Entry* whole;          // whole program, for blocks whose pruning overflowed
+	uint32_t n_whole;
+	int waves;                   // waves per workgroup
+};
+
 // Launchers, one set per numerics build (exact = reference bit parity,
 // fast = contracted FP32).  `p` is points per lane for the tile kernel (4 or 8).
 namespace exact {
 hipError_t launch_points(const PointArgs& a, hipStream_t s);
 hipError_t launch_tiles(const GridArgs& a, int p, hipStream_t s);
+hipError_t launch_prune_blocks(const PruneArgs& a, hipStream_t s);
+hipError_t launch_eval_blocks(const BlockEvalArgs& a, int p, hipStream_t s);
 }
 namespace fast {
 hipError_t launch_points(const PointArgs& a, hipStream_t s);
 hipError_t launch_tiles(const GridArgs& a, int p, hipStream_t s);
+hipError_t launch_prune_blocks(const PruneArgs& a, hipStream_t s);
+hipError_t launch_eval_blocks(const BlockEvalArgs& a, int p, hipStream_t s);
 }
 
-// Dynamic LDS bytes per wave for the interpreter spill region.
-inline size_t spill_bytes_per_wave(uint32_t slots, int points_per_lane)
+// LDS layout of the block kernel (bytes), shared by launcher and kernel.
+constexpr uint32_t kMaxBlockBinary = 1024;     // decision masks held for a block program
+constexpr uint32_t kTileListCap = 256;         // entries of a pruned tile program
+constexpr uint32_t kDfsCap = 128;              // DFS frames per wave
+__host__ __device__ inline size_t block_lds_bytes(uint32_t slots, int p, int waves)
 {
-	return size_t(slots) * size_t(points_per_lane) * 64 * sizeof(float);
+	return size_t(kMaxBlockBinary) * 16 + size_t(waves) * (size_t(kTileListCap) * 16 + size_t(kDfsCap) * 16 +
+	                                                        spill_bytes_per_wave(slots, p));
+}
+__host__ __device__ inline size_t prune_lds_bytes(uint32_t n_binary, uint32_t slots)
+{
+	return size_t(n_binary) * 16 + size_t(kDfsCap) * 16 + spill_bytes_per_wave(slots, 1);
 }
 
 }  // namespace tg

@@ -1,5 +1,6 @@
 #include "tg_program.h"
 
+#include <cmath>
 #include <cstring>
 
 namespace tg {
@@ -34,6 +35,29 @@ int brush_param_count(uint32_t op)
 bool is_brush(uint32_t op) { return op >= OP_SPHERE && op <= OP_PLANE; }
 bool is_binary(uint32_t op) { return op >= OP_UNION && op <= OP_BLEND_DIFF; }
 bool is_blend(uint32_t op) { return op >= OP_BLEND_UNION && op <= OP_BLEND_DIFF; }
+
+// Largest singular value of the 3x3 part of a record's point transform
+// (power iteration on M^T M, double precision).
+double spectral_norm3(const float* rows)
+{
+	double a[3][3] = {};
+	for (int i = 0; i < 3; ++i)
+		for (int j = 0; j < 3; ++j)
+			for (int k = 0; k < 3; ++k) a[i][j] += double(rows[k * 4 + i]) * double(rows[k * 4 + j]);
+	double v[3] = { 1.0, 0.7548776662, 0.5698402910 };
+	double lambda = 0.0;
+	for (int it = 0; it < 64; ++it)
+	{
+		double w[3] = { 0, 0, 0 };
+		for (int i = 0; i < 3; ++i)
+			for (int j = 0; j < 3; ++j) w[i] += a[i][j] * v[j];
+		const double n = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+		if (n == 0.0) return 0.0;
+		for (int i = 0; i < 3; ++i) v[i] = w[i] / n;
+		lambda = n;
+	}
+	return std::sqrt(lambda);
+}
 
 }  // namespace
 
@@ -71,11 +95,17 @@ bool decode_words(const uint32_t* w, size_t n, DecodedProgram& out, std::string&
 	uint32_t pending_kind = XF_NONE;
 	float pending[12] = { 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0 };
 	std::vector<uint32_t> starts;  // subtree start record per stack entry
+	double last_brush_norm = 1.0;  // transform norm of the brush just emitted
 	size_t pc = 0;
 	bool stopped = false;
 	while (pc < n)
 	{
 		const uint32_t op = w[pc++];
+		if (op != OP_SCALE_FIELD)
+		{
+			if (last_brush_norm > 1.0 + 1e-5) out.lipschitz = false;
+			last_brush_norm = 1.0;
+		}
 		if (op == OP_STOP)
 		{
 			if (pc != n)
@@ -110,7 +140,17 @@ bool decode_words(const uint32_t* w, size_t n, DecodedProgram& out, std::string&
 			starts.push_back(uint32_t(out.instrs.size()));
 			out.instrs.push_back(b);
 			out.brushes++;
+			// Pruning relies on every brush being a 1-Lipschitz field: exact
+			// distance functions seen through a rigid (rotation + uniform scale,
+			// undone by ScaleField) transform.
 			if (op == OP_ELLIPSOID) out.lipschitz = false;
+			if (op == OP_PLANE)
+			{
+				const double nx = as_float(b.w[13]), ny = as_float(b.w[14]), nz = as_float(b.w[15]);
+				if (std::sqrt(nx * nx + ny * ny + nz * nz) > 1.0 + 1e-5) out.lipschitz = false;
+			}
+			// checked at the next record, which may be the ScaleField undoing a scale
+			last_brush_norm = (pending_kind == XF_MATRIX) ? spectral_norm3(pending) : 1.0;
 			out.max_stack = std::max<uint32_t>(out.max_stack, uint32_t(starts.size()));
 			pending_kind = XF_NONE;
 			const float ident[12] = { 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0 };
@@ -159,6 +199,13 @@ bool decode_words(const uint32_t* w, size_t n, DecodedProgram& out, std::string&
 			Instr u{};
 			u.w[0] = op;
 			u.w[1] = w[pc++];
+			if (op == OP_SCALE_FIELD)
+			{
+				// field scale s after a transform of norm n: Lipschitz n * |s|
+				const double ls = last_brush_norm * std::fabs(double(as_float(u.w[1])));
+				if (ls > 1.0 + 1e-5) out.lipschitz = false;
+				last_brush_norm = 1.0;
+			}
 			out.instrs.push_back(u);
 			continue;
 		}
@@ -206,7 +253,36 @@ bool decode_words(const uint32_t* w, size_t n, DecodedProgram& out, std::string&
 		      std::to_string(kMaxStack);
 		return false;
 	}
-	for (const Instr& in : out.instrs) out.flops += instr_flops(in);
+	// Record cost in bits 16..23 of word 0 (device-side work accounting), and
+	// the entry view with operand roots for per-cell pruning.
+	uint32_t ord = 0;
+	for (size_t i = 0; i < out.instrs.size(); ++i)
+	{
+		Instr& in = out.instrs[i];
+		const double c = instr_flops(in);
+		out.flops += c;
+		in.w[0] = (in.w[0] & 0xffffu) | (uint32_t(c) << 16);
+		const uint32_t op = instr_op(in);
+		if (op == OP_STOP) break;
+		HostEntry e{ uint32_t(i) | (op << 24), 0, 0, 0 };
+		if (is_binary(op))
+		{
+			e.l = in.w[2] - 1;   // the left operand ends right before the right one starts
+			e.r = uint32_t(i) - 1;
+			e.ord = ord++;
+		}
+		else if (op == OP_FLATE || op == OP_SCALE_FIELD)
+		{
+			e.l = uint32_t(i) - 1;
+		}
+		out.entries.push_back(e);
+	}
+	out.binary_entries = ord;
+	if (out.instrs.size() >= (1u << 24))
+	{
+		err = "program exceeds 2^24 records";
+		return false;
+	}
 	return true;
 }
 

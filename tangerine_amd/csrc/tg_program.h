@@ -9,9 +9,17 @@
 
 namespace tg {
 
+// Entry view of a program (see Entry in tg_kernels.h): rec | op << 24, l, r, ord.
+struct HostEntry
+{
+	uint32_t rec, l, r, ord;
+};
+
 struct DecodedProgram
 {
 	std::vector<Instr> instrs;   // ends with OP_STOP
+	std::vector<HostEntry> entries;  // every record but Stop, in order
+	uint32_t binary_entries = 0;
 	uint32_t max_stack = 0;      // interpreter stack high-water mark
 	uint32_t brushes = 0;
 	uint32_t set_ops = 0;

@@ -1,0 +1,82 @@
+"""Grid export on the GPU: the MeshExportThread shape (tangerine/export.cpp:320-381)
+behind tg_mesher_* (evaluate every grid sample, then naive surface nets, all on
+the device)."""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+
+
+class Mesher:
+    """Owns one device stream and the export buffers (tg_mesher)."""
+
+    def __init__(self):
+        self.lib = _lib.load()
+        h = C.c_void_p()
+        check(self.lib.tg_mesher_create(C.byref(h)), "tg_mesher_create")
+        self.h = h.value
+        self._cells = None
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.tg_mesher_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def export(self, program, origin, step, cells, fast: bool = False, prune: bool = True) -> dict:
+        g = _lib.grid_struct(origin, step, cells)
+        flags = (_lib.TG_FLAG_FAST if fast else 0) | (0 if prune else _lib.TG_FLAG_NO_PRUNE)
+        check(self.lib.tg_mesher_export(self.h, program.handle, C.byref(g), flags), "tg_mesher_export")
+        self._cells = tuple(int(c) for c in cells)
+        return self.stats()
+
+    def mesh_values(self, values, origin, step, cells) -> dict:
+        g = _lib.grid_struct(origin, step, cells)
+        v = np.ascontiguousarray(values, dtype=np.float32)
+        check(self.lib.tg_mesher_mesh_values(self.h, v.ctypes.data_as(_lib._fp), C.byref(g)), "tg_mesher_mesh_values")
+        self._cells = tuple(int(c) for c in cells)
+        return self.stats()
+
+    def stats(self) -> dict:
+        st = _lib.TgExportStats()
+        check(self.lib.tg_mesher_stats(self.h, C.byref(st)), "tg_mesher_stats")
+        return {k: getattr(st, k) for k, _ in _lib.TgExportStats._fields_}
+
+    def counts(self) -> tuple[int, int]:
+        nv, nt = C.c_uint64(), C.c_uint64()
+        check(self.lib.tg_mesher_counts(self.h, C.byref(nv), C.byref(nt)), "tg_mesher_counts")
+        return nv.value, nt.value
+
+    def mesh(self) -> tuple[np.ndarray, np.ndarray]:
+        nv, nt = self.counts()
+        v = np.zeros((nv, 3), dtype=np.float32)
+        t = np.zeros((nt, 3), dtype=np.uint32)
+        check(self.lib.tg_mesher_copy_mesh(self.h, v.ctypes.data_as(_lib._fp), t.ctypes.data_as(_lib._u32p)),
+              "tg_mesher_copy_mesh")
+        return v, t
+
+    def grid(self) -> np.ndarray:
+        nx, ny, nz = (c + 1 for c in self._cells)
+        out = np.zeros(nx * ny * nz, dtype=np.float32)
+        check(self.lib.tg_mesher_copy_grid(self.h, out.ctypes.data_as(_lib._fp)), "tg_mesher_copy_grid")
+        return out
+
+
+def export_grid_for_bounds(bounds: np.ndarray, step: float):
+    """Grid of MeshExportThread (tangerine/export.cpp:324-337): the lower bound
+    moved down by two steps, ceil((max - min) / step) cells per axis."""
+    lo = np.asarray(bounds[0], dtype=np.float32) - np.float32(step) * np.float32(2.0)
+    hi = np.asarray(bounds[1], dtype=np.float32)
+    cells = tuple(int(math.ceil(float(np.float32((hi[i] - lo[i]) / np.float32(step))))) for i in range(3))
+    return tuple(float(v) for v in lo), (float(np.float32(step)),) * 3, cells

@@ -138,6 +138,10 @@ class SDF:
               "tg_node_compile")
         return words
 
+    def inspect(self) -> dict:
+        """Program properties from the host decoder (no device needed)."""
+        return inspect_words(self.compile())
+
     def program(self) -> "Program":
         if self._program is None:
             self._program = Program.from_node(self)
@@ -153,6 +157,24 @@ class SDF:
         return self.program().eval_points(points, fast=fast)
 
 
+def _info_dict(info) -> dict:
+    return {
+        "instructions": int(info.instructions),
+        "max_stack": int(info.max_stack),
+        "brushes": int(info.brushes),
+        "set_ops": int(info.set_ops),
+        "lipschitz": bool(info.lipschitz),
+        "flops_per_eval": float(info.flops_per_eval),
+    }
+
+
+def inspect_words(words) -> dict:
+    w = np.ascontiguousarray(words, dtype=np.uint32)
+    info = _lib.TgProgramInfo()
+    check(_lib.load().tg_words_inspect(w.ctypes.data_as(_lib._u32p), w.size, C.byref(info)), "tg_words_inspect")
+    return _info_dict(info)
+
+
 class Program:
     """A flattened tree resident in device memory (tg_program)."""
 
@@ -162,14 +184,7 @@ class Program:
         self._h = handle
         info = _lib.TgProgramInfo()
         check(_lib.load().tg_program_info_get(handle, C.byref(info)), "tg_program_info_get")
-        self.info = {
-            "instructions": int(info.instructions),
-            "max_stack": int(info.max_stack),
-            "brushes": int(info.brushes),
-            "set_ops": int(info.set_ops),
-            "lipschitz": bool(info.lipschitz),
-            "flops_per_eval": float(info.flops_per_eval),
-        }
+        self.info = _info_dict(info)
 
     @classmethod
     def from_node(cls, node: SDF) -> "Program":
