@@ -35,6 +35,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 from tangerine_amd import _lib, scenes  # noqa: E402
+from tangerine_amd.export import Mesher  # noqa: E402
 
 PEAK_FP32_TFLOPS = 157.3          # MI355X published FP32 vector peak (packed issue)
 PEAK_FP32_NONPACKED_TFLOPS = 78.6  # same without packed FP32 (not used by these kernels)
@@ -48,28 +49,6 @@ WORKLOADS = {
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
-
-
-class Mesher:
-    def __init__(self, lib):
-        self.lib = lib
-        h = C.c_void_p()
-        _lib.check(lib.tg_mesher_create(C.byref(h)), "tg_mesher_create")
-        self.h = h.value
-
-    def export(self, prog, grid, flags):
-        _lib.check(self.lib.tg_mesher_export(self.h, prog.handle, C.byref(grid), flags), "tg_mesher_export")
-        st = _lib.TgExportStats()
-        _lib.check(self.lib.tg_mesher_stats(self.h, C.byref(st)), "tg_mesher_stats")
-        return st
-
-    def counts(self):
-        nv, nt = C.c_uint64(), C.c_uint64()
-        _lib.check(self.lib.tg_mesher_counts(self.h, C.byref(nv), C.byref(nt)), "counts")
-        return nv.value, nt.value
-
-    def close(self):
-        self.lib.tg_mesher_destroy(self.h)
 
 
 def cpu_baseline(spec, lo, hi, cells, budget_s: float, threads: int) -> dict:
@@ -129,6 +108,7 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="export1024", choices=sorted(WORKLOADS))
     ap.add_argument("--fast", action="store_true", help="contracted FP32 numerics instead of bit parity")
+    ap.add_argument("--no-prune", action="store_true", help="interpret the whole program in every tile")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("TG_CPU_THREADS", "16")))
@@ -149,25 +129,27 @@ def main(argv=None) -> int:
     node = scenes.build_sdf(spec)
     prog = node.program()
     origin, step, cells = scenes.grid_for(wl["lo"], wl["hi"], wl["cells"])
-    grid = _lib.grid_struct(origin, step, cells)
-    flags = _lib.TG_FLAG_FAST if args.fast else 0
-    mesher = Mesher(lib)
+    mesher = Mesher()
+    prune = not args.no_prune
     log(f"workload {args.workload}: {prog.info}")
 
+    def step_once():
+        return mesher.export(prog, origin, step, cells, fast=args.fast, prune=prune)
+
     for i in range(args.warmup):
-        st = mesher.export(prog, grid, flags)
-        log(f"warmup {i}: eval {st.eval_ms:.1f} ms mesh {st.mesh_ms:.1f} ms")
+        st = step_once()
+        log(f"warmup {i}: eval {st['eval_ms']:.1f} ms mesh {st['mesh_ms']:.1f} ms")
     _lib.check(lib.tg_device_synchronize(), "sync")
     evals_ms, mesh_ms, totals = [], [], []
     flops = evals = 0
     t0 = time.perf_counter()
     for i in range(args.steps):
-        st = mesher.export(prog, grid, flags)
-        evals_ms.append(st.eval_ms)
-        mesh_ms.append(st.mesh_ms)
-        totals.append(st.total_ms)
-        flops, evals = st.flops, st.exact_evals
-        log(f"step {i}: eval {st.eval_ms:.1f} ms mesh {st.mesh_ms:.1f} ms total {st.total_ms:.1f} ms")
+        st = step_once()
+        evals_ms.append(st["eval_ms"])
+        mesh_ms.append(st["mesh_ms"])
+        totals.append(st["total_ms"])
+        flops, evals = st["flops"], st["exact_evals"]
+        log(f"step {i}: eval {st['eval_ms']:.1f} ms mesh {st['mesh_ms']:.1f} ms total {st['total_ms']:.1f} ms")
     _lib.check(lib.tg_device_synchronize(), "sync")
     wall = time.perf_counter() - t0
     nverts, ntris = mesher.counts()
@@ -194,6 +176,7 @@ def main(argv=None) -> int:
             "workload": f"{args.workload}: plain exact export (grid eval + naive surface nets)",
             "primitives": wl["primitives"], "seed": wl["seed"], "cells": wl["cells"],
             "bounds": [wl["lo"], wl["hi"]], "samples": samples, "numerics": "fast" if args.fast else "exact",
+            "pruning": "per-brick (32^3 blocks, 8^3 tiles)" if prune else "none",
         },
         "export_ms_plain": float(np.mean(totals)),
         "eval_ms": eval_ms,
@@ -204,7 +187,7 @@ def main(argv=None) -> int:
             "bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
             "frac_of_nonpacked_peak": achieved / PEAK_FP32_NONPACKED_TFLOPS,
-            "kernel": "k_eval_tiles", "flops_per_launch": flops, "launch_ms": eval_ms,
+            "kernel": "k_eval_blocks" if prune else "k_eval_tiles", "flops_per_launch": flops, "launch_ms": eval_ms,
         },
     }
     if not args.no_cpu:

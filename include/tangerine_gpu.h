@@ -57,6 +57,30 @@ extern "C" {
 typedef struct tg_node tg_node;
 typedef struct tg_program tg_program;
 typedef struct tg_mesher tg_mesher;
+typedef struct tg_tracer tg_tracer;
+
+/* Pinhole camera: pixel (i, j) looks along
+ *   normalize(forward + sx*tan_half_x*right + sy*tan_half_y*up),
+ *   sx = (i + .5) * (2/width) - 1, sy = 1 - (j + .5) * (2/height). */
+typedef struct tg_camera
+{
+	float eye[3];
+	float forward[3];
+	float right[3];
+	float up[3];
+	float tan_half_x, tan_half_y;
+	uint32_t width, height;
+} tg_camera;
+
+typedef struct tg_trace_stats
+{
+	double trace_ms;           /* tracing kernel (HIP events) */
+	double prune_ms;           /* cell-program pruning (HIP events) */
+	uint64_t rays;
+	uint64_t hits;
+	uint64_t evaluations;      /* exact evaluations, march steps and normal taps */
+	double flops;              /* algorithmic FP32 work executed (cost table) */
+} tg_trace_stats;
 
 /* Same fields and meaning as isosurface::regular_grid_t
  * (third_party/naive-surface-nets/include/regular_grid.h): origin, cell
@@ -147,6 +171,37 @@ int tg_eval_points(const tg_program* program, const float* xyz, uint64_t count, 
 /* Same with device buffers on `stream` (a hipStream_t, or NULL for the null stream). */
 int tg_eval_points_device(const tg_program* program, const float* xyz, uint64_t count, float* out,
                           uint32_t flags, void* stream);
+
+/* Gradients at points.  mode 0: SDFNode::Gradient's tetrahedral rule with its
+ * fallback (sdf_evaluator.cpp:298-333; pass h = 1e-4 for the reference step);
+ * mode 1: axis central differences with step h.  out: 3 floats per point. */
+int tg_gradient_points(const tg_program* program, const float* xyz, uint64_t count, float h, int mode,
+                       float* out, uint32_t flags);
+
+/* ---- sphere tracing (SDFNode::RayMarch, batched) ------------------------- */
+/* Rays from host arrays (directions are normalized as glm::normalize does);
+ * far = INFINITY reproduces RayMarch exactly.  hit: 1 byte per ray; travel is
+ * INFINITY on a miss; position: 3 floats per ray (last position on a miss). */
+int tg_raymarch(const tg_program* program, const float* starts, const float* dirs, uint64_t count,
+                int max_steps, float epsilon, float far_distance, uint32_t flags,
+                uint8_t* hit, float* travel, float* position);
+/* Camera looking from eye at target; up is replaced when parallel to the view. */
+int tg_camera_look_at(tg_camera* out, const float eye[3], const float target[3], const float up[3],
+                      float fov_y_degrees, uint32_t width, uint32_t height);
+int tg_tracer_create(tg_tracer** out);
+void tg_tracer_destroy(tg_tracer* tracer);
+/* Cube [lo, lo + size]^3 over which per-cell programs are built; positions
+ * outside it run the whole program.  Default: the origin-centred cube holding
+ * the eye. */
+int tg_tracer_set_domain(tg_tracer* tracer, const float lo[3], float size);
+/* Renders width*height rays on the device; hit normals by central differences
+ * with step normal_h (<= 0: none).  Results stay on the device until copied. */
+int tg_tracer_render(tg_tracer* tracer, const tg_program* program, const tg_camera* camera, int max_steps,
+                     float epsilon, float far_distance, float normal_h, uint32_t flags);
+/* t: 1 float per pixel (INFINITY = miss); normals: 3 floats per pixel or NULL;
+ * steps: 1 uint32 per pixel or NULL.  Row-major, row 0 at the top. */
+int tg_tracer_copy(const tg_tracer* tracer, float* t, float* normals, uint32_t* steps);
+int tg_tracer_stats(const tg_tracer* tracer, tg_trace_stats* out);
 
 /* ---- grid export (MeshExportThread shape: evaluate, then naive surface nets) */
 int tg_mesher_create(tg_mesher** out);

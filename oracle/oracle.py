@@ -161,6 +161,8 @@ class Oracle(_Cpu):
         L.tgo_grid_interp.argtypes = [_u32p, _u64, _fp, _fp, _u64p, _u64, _u64, C.c_int, _fp]
         L.tgo_gradient_central_words.restype = C.c_int
         L.tgo_gradient_central_words.argtypes = [_u32p, _u64, _fp, _u64, _f, _fp]
+        L.tgo_gradient_tetra_words.restype = C.c_int
+        L.tgo_gradient_tetra_words.argtypes = [_u32p, _u64, _fp, _u64, _f, _fp]
         L.tgo_raymarch_far_words.restype = C.c_int
         L.tgo_raymarch_far_words.argtypes = [_u32p, _u64, _fp, _fp, _u64, C.c_int, _f, _f, _u8p, _fp, _i32p]
         L.tgo_surface_nets.restype = _p
@@ -194,6 +196,15 @@ class Oracle(_Cpu):
         out = np.empty_like(pts)
         if self.lib.tgo_gradient_central_words(w.ctypes.data_as(_u32p), w.size, _fptr(pts), pts.shape[0],
                                                float(h), _fptr(out)) != 0:
+            raise ValueError("malformed program")
+        return out
+
+    def gradient_tetra(self, words, points, h=1e-4) -> np.ndarray:
+        w = np.ascontiguousarray(words, dtype=np.uint32)
+        pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        out = np.empty_like(pts)
+        if self.lib.tgo_gradient_tetra_words(w.ctypes.data_as(_u32p), w.size, _fptr(pts), pts.shape[0], float(h),
+                                             _fptr(out)) != 0:
             raise ValueError("malformed program")
         return out
 

@@ -920,6 +920,42 @@ int tgo_gradient_central_words(const uint32_t* words, uint64_t nwords, const flo
 	return 0;
 }
 
+// SDFNode::Gradient's tetrahedral rule (sdf_evaluator.cpp:298-333) over the
+// interpreter instead of the tree walk.
+int tgo_gradient_tetra_words(const uint32_t* words, uint64_t nwords, const float* p, uint64_t n, float h, float* out)
+{
+	for (uint64_t i = 0; i < n; ++i)
+	{
+		V3 c = { p[3 * i], p[3 * i + 1], p[3 * i + 2] };
+		V2 o = { 1.0f * h, -1.0f * h };
+		V3 kxyy = { o.x, o.y, o.y }, kyyx = { o.y, o.y, o.x }, kyxy = { o.y, o.x, o.y }, kxxx = { o.x, o.x, o.x };
+		bool ok = true, k1, k2, k3, k4;
+		float e1 = interpret(words, nwords, add3(c, kxyy), &k1);
+		float e2 = interpret(words, nwords, add3(c, kyyx), &k2);
+		float e3 = interpret(words, nwords, add3(c, kyxy), &k3);
+		float e4 = interpret(words, nwords, add3(c, kxxx), &k4);
+		ok = k1 && k2 && k3 && k4;
+		V3 g = add3(add3(add3(mul3s(kxyy, e1), mul3s(kyyx, e2)), mul3s(kyxy, e3)), mul3s(kxxx, e4));
+		float l2 = dot3(g, g);
+		V3 r;
+		if (l2 == 0.0f)
+		{
+			float d0 = interpret(words, nwords, c, &k1);
+			V3 v = { interpret(words, nwords, add3(c, kxyy), &k2) - d0, interpret(words, nwords, add3(c, kyxy), &k3) - d0,
+				interpret(words, nwords, add3(c, kyyx), &k4) - d0 };
+			ok = ok && k1 && k2 && k3 && k4;
+			r = mul3s(v, 1.0f / std::sqrt(dot3(v, v)));
+		}
+		else
+		{
+			r = div3s(g, std::sqrt(l2));
+		}
+		if (!ok) return -1;
+		out[3 * i] = r.x; out[3 * i + 1] = r.y; out[3 * i + 2] = r.z;
+	}
+	return 0;
+}
+
 // SDFNode::RayMarch (sdf_evaluator.cpp:336-353) on the tree.
 void tgo_raymarch(void* a, const float* start, const float* dir, uint64_t n, int iters, float eps,
 	uint8_t* hit, float* travel, float* pos)

@@ -41,6 +41,16 @@ class TgExportStats(C.Structure):
                 ("samples", _u64), ("tiles_evaluated", _u64), ("exact_evals", _u64), ("flops", C.c_double)]
 
 
+class TgCamera(C.Structure):
+    _fields_ = [("eye", _f * 3), ("forward", _f * 3), ("right", _f * 3), ("up", _f * 3),
+                ("tan_half_x", _f), ("tan_half_y", _f), ("width", _u32), ("height", _u32)]
+
+
+class TgTraceStats(C.Structure):
+    _fields_ = [("trace_ms", C.c_double), ("prune_ms", C.c_double), ("rays", _u64), ("hits", _u64),
+                ("evaluations", _u64), ("flops", C.c_double)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "tg_last_error": (C.c_char_p, []),
@@ -81,6 +91,15 @@ SIGNATURES = {
     "tg_words_inspect": (C.c_int, [_u32p, _u64, C.POINTER(TgProgramInfo)]),
     "tg_eval_points": (C.c_int, [_p, _fp, _u64, _fp, _u32]),
     "tg_eval_points_device": (C.c_int, [_p, _p, _u64, _p, _u32, _p]),
+    "tg_gradient_points": (C.c_int, [_p, _fp, _u64, _f, C.c_int, _fp, _u32]),
+    "tg_raymarch": (C.c_int, [_p, _fp, _fp, _u64, C.c_int, _f, _f, _u32, C.POINTER(C.c_uint8), _fp, _fp]),
+    "tg_camera_look_at": (C.c_int, [C.POINTER(TgCamera), _fp, _fp, _fp, _f, _u32, _u32]),
+    "tg_tracer_create": (C.c_int, [C.POINTER(_p)]),
+    "tg_tracer_destroy": (None, [_p]),
+    "tg_tracer_set_domain": (C.c_int, [_p, _fp, _f]),
+    "tg_tracer_render": (C.c_int, [_p, _p, C.POINTER(TgCamera), C.c_int, _f, _f, _f, _u32]),
+    "tg_tracer_copy": (C.c_int, [_p, _fp, _fp, _u32p]),
+    "tg_tracer_stats": (C.c_int, [_p, C.POINTER(TgTraceStats)]),
     "tg_mesher_create": (C.c_int, [C.POINTER(_p)]),
     "tg_mesher_destroy": (None, [_p]),
     "tg_mesher_export": (C.c_int, [_p, _p, C.POINTER(TgGrid), _u32]),

@@ -41,6 +41,8 @@ HIP_UNITS = [
     ("tg_mesh.hip", ["-ffp-contract=off"]),
     ("tg_capi.hip", ["-ffp-contract=off"]),
     ("tg_export.hip", ["-ffp-contract=off"]),
+    ("tg_prune.hip", ["-ffp-contract=off"]),
+    ("tg_trace.hip", ["-ffp-contract=off"]),
 ]
 
 HOST_UNITS = ["tg_tree.cpp", "tg_program.cpp"]

@@ -36,4 +36,23 @@ struct tg_program
 
 namespace tg {
 int upload_program(const DecodedProgram& d, tg_program** out);
+
+// Level-1 pruned programs of a cell grid (BlockDesc per cell + entry pool).
+struct CellPrograms
+{
+	DevBuf pool, desc, counters;
+	uint64_t ncells = 0;
+	uint64_t entries = 0;     // pool entries in use
+	uint32_t max_stack = 0;   // over all cells
+	bool valid = false;       // false: the program is not prunable here
+	void release()
+	{
+		pool.release();
+		desc.release();
+		counters.release();
+	}
+};
+
+int prune_cells(hipStream_t stream, const tg_program* p, const float origin[3], const float cell[3], float radius,
+	const uint32_t counts[3], bool fast, CellPrograms& out);
 }

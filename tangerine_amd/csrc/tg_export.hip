@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 
@@ -29,7 +30,8 @@ struct tg_mesher
 	hipStream_t stream = nullptr;
 	hipEvent_t ev[4] = {};
 	DevBuf pool, signs, active, word_count, word_base, quad_count, quad_base, vertices, triangles, scan_tmp, dense;
-	DevBuf block_pool, block_desc, counters;
+	DevBuf counters;
+	CellPrograms blocks;
 	MeshArgs args{};
 	uint64_t nverts = 0, ntris = 0;
 	bool have_grid = false;
@@ -130,83 +132,60 @@ int surface_nets(tg_mesher* m)
 	return TG_OK;
 }
 
-constexpr size_t kMaxBlockPoolBytes = size_t(8) << 30;
 constexpr size_t kMaxLds = 160 * 1024;
 
-// Two-level pruned evaluation of every tile (see k_prune_blocks / k_eval_blocks).
-// Returns 1 if it ran, 0 if the program or grid is outside its limits (the
-// caller then interprets the whole program per tile), <0 on error.
+// Two-level pruned evaluation of every tile (k_prune_cells over 32^3-sample
+// blocks, then k_eval_blocks).  Returns 1 if it ran, 0 if the program is not
+// prunable (the caller then interprets the whole program per tile), <0 on
+// error.
 int eval_pruned(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t flags, const uint64_t* tile_mask,
 	const uint32_t* block_list, uint32_t n_listed, uint64_t* evals, double* flops)
 {
-	const DecodedProgram& d = p->decoded;
 	MeshArgs& ma = m->args;
-	if (!d.lipschitz || d.entries.empty() || d.entries.size() >= 65535) return 0;
-	const uint32_t bx = (ma.tx + kBlockTiles - 1) / kBlockTiles;
-	const uint32_t by = (ma.ty + kBlockTiles - 1) / kBlockTiles;
-	const uint32_t bz = (ma.tz + kBlockTiles - 1) / kBlockTiles;
-	const uint64_t nblocks = uint64_t(bx) * by * bz;
-	const uint32_t cap = uint32_t(d.entries.size());
-	const uint32_t whole_slots = std::max<uint32_t>(1, d.max_stack - 1);
-	if (nblocks * cap * sizeof(Entry) > kMaxBlockPoolBytes) return 0;
-	if (prune_lds_bytes(d.binary_entries, whole_slots) > 64 * 1024) return 0;
-	int rc = m->block_pool.ensure(nblocks * cap * sizeof(Entry));
-	if (!rc) rc = m->block_desc.ensure(nblocks * sizeof(BlockDesc));
-	if (!rc) rc = m->counters.ensure(64);
+	const uint32_t counts[3] = { (ma.tx + kBlockTiles - 1) / kBlockTiles, (ma.ty + kBlockTiles - 1) / kBlockTiles,
+	                             (ma.tz + kBlockTiles - 1) / kBlockTiles };
+	// Block (32 samples per axis) centred at g + (32 b + 15.5) d: a cell grid with
+	// origin g - d/2 and cell 32 d; the samples lie within 15.5 |d| of the centre.
+	const float origin[3] = { g->x - 0.5f * g->dx, g->y - 0.5f * g->dy, g->z - 0.5f * g->dz };
+	const float cell[3] = { float(kBlockSamples) * g->dx, float(kBlockSamples) * g->dy, float(kBlockSamples) * g->dz };
+	const float half = 0.5f * float(kBlockSamples - 1);
+	const float radius = half * std::sqrt(g->dx * g->dx + g->dy * g->dy + g->dz * g->dz);
+	int rc = prune_cells(m->stream, p, origin, cell, radius, counts, (flags & TG_FLAG_FAST) != 0, m->blocks);
 	if (rc) return rc;
-	uint32_t* max_stack = m->counters.as<uint32_t>();
-	unsigned long long* work = reinterpret_cast<unsigned long long*>(m->counters.as<unsigned char>() + 16);
+	if (!m->blocks.valid) return 0;
+	rc = m->counters.ensure(64);
+	if (rc) return rc;
+	unsigned long long* work = m->counters.as<unsigned long long>();
 	TG_CHECK_HIP(hipMemsetAsync(m->counters.ptr, 0, 64, m->stream), "counter reset");
 
-	PruneArgs pa{};
-	pa.prog = static_cast<const Instr*>(p->device.ptr);
-	pa.entries = p->entries.as<const Entry>();
-	pa.n_entries = cap;
-	pa.n_binary = d.binary_entries;
-	pa.gx = g->x; pa.gy = g->y; pa.gz = g->z;
-	pa.dx = g->dx; pa.dy = g->dy; pa.dz = g->dz;
-	pa.bx = bx; pa.by = by; pa.bz = bz;
-	pa.slots = whole_slots;
-	pa.whole_flops = uint32_t(d.flops);
-	pa.pool = m->block_pool.as<Entry>();
-	pa.cap = cap;
-	pa.desc = m->block_desc.as<BlockDesc>();
-	pa.max_stack = max_stack;
-	hipError_t e = (flags & TG_FLAG_FAST) ? fast::launch_prune_blocks(pa, m->stream)
-	                                      : exact::launch_prune_blocks(pa, m->stream);
-	if (e != hipSuccess) return hip_fail(e, "block pruning launch");
-	uint32_t block_stack = 0;
-	TG_CHECK_HIP(hipMemcpyAsync(&block_stack, max_stack, 4, hipMemcpyDeviceToHost, m->stream), "stack read");
-	TG_CHECK_HIP(hipStreamSynchronize(m->stream), "block pruning");
-
 	BlockEvalArgs ba{};
-	ba.prog = pa.prog;
-	ba.pool = pa.pool;
-	ba.desc = pa.desc;
+	ba.prog = static_cast<const Instr*>(p->device.ptr);
+	ba.pool = m->blocks.pool.as<const Entry>();
+	ba.desc = m->blocks.desc.as<const BlockDesc>();
 	ba.block_list = block_list;
 	ba.tile_mask = tile_mask;
-	ba.nblocks = block_list ? n_listed : uint32_t(nblocks);
+	ba.nblocks = block_list ? n_listed : uint32_t(m->blocks.ncells);
 	ba.gx = g->x; ba.gy = g->y; ba.gz = g->z;
 	ba.dx = g->dx; ba.dy = g->dy; ba.dz = g->dz;
 	ba.nx = ma.nx; ba.ny = ma.ny; ba.nz = ma.nz;
 	ba.tx = ma.tx; ba.ty = ma.ty; ba.tz = ma.tz;
-	ba.bx = bx; ba.by = by; ba.bz = bz;
+	ba.bx = counts[0]; ba.by = counts[1]; ba.bz = counts[2];
 	ba.tile_slot = ma.tile_slot;
 	ba.out_pool = m->pool.as<float>();
 	ba.signs = m->signs.as<uint8_t>();
 	ba.row_bytes = ma.row_bytes;
-	ba.slots = std::max<uint32_t>(1, block_stack > 0 ? block_stack - 1 : 1);
+	ba.slots = std::max<uint32_t>(1, m->blocks.max_stack > 0 ? m->blocks.max_stack - 1 : 1);
 	ba.prune_tiles = 1;
 	ba.work = work;
-	ba.whole = pa.entries;
-	ba.n_whole = cap;
+	ba.whole = p->entries.as<const Entry>();
+	ba.n_whole = uint32_t(p->decoded.entries.size());
 	ba.waves = 4;
 	int pts = m->points_per_lane;
 	while (pts > 2 && block_lds_bytes(ba.slots, pts, ba.waves) > m->lds_budget) pts >>= 1;
 	if (block_lds_bytes(ba.slots, pts, ba.waves) > kMaxLds) return 0;
 	m->last_points_per_lane = pts;
-	e = (flags & TG_FLAG_FAST) ? fast::launch_eval_blocks(ba, pts, m->stream)
-	                           : exact::launch_eval_blocks(ba, pts, m->stream);
+	hipError_t e = (flags & TG_FLAG_FAST) ? fast::launch_eval_blocks(ba, pts, m->stream)
+	                                      : exact::launch_eval_blocks(ba, pts, m->stream);
 	if (e != hipSuccess) return hip_fail(e, "block evaluation launch");
 	unsigned long long wk[2] = { 0, 0 };
 	TG_CHECK_HIP(hipMemcpyAsync(wk, work, 16, hipMemcpyDeviceToHost, m->stream), "work read");
@@ -240,9 +219,9 @@ extern "C" void tg_mesher_destroy(tg_mesher* m)
 	if (!m) return;
 	if (m->stream) hipStreamSynchronize(m->stream);
 	for (DevBuf* b : { &m->pool, &m->signs, &m->active, &m->word_count, &m->word_base, &m->quad_count, &m->quad_base,
-	                   &m->vertices, &m->triangles, &m->scan_tmp, &m->dense, &m->block_pool, &m->block_desc,
-	                   &m->counters })
+	                   &m->vertices, &m->triangles, &m->scan_tmp, &m->dense, &m->counters })
 		b->release();
+	m->blocks.release();
 	for (hipEvent_t e : m->ev)
 		if (e) hipEventDestroy(e);
 	if (m->stream) hipStreamDestroy(m->stream);

@@ -66,21 +66,28 @@ struct BlockDesc
 	uint32_t flops;      // cost-table work of one evaluation
 };
 
-// Level 1: prune the whole program at every block centre.
+// Level 1: prune the whole program for every cell of a regular cell grid
+// (cell (i,j,k) centred at origin + (i+.5, j+.5, k+.5) * cell, valid within
+// `radius` of its centre).  Pruned programs are packed into `pool`; a cell
+// whose program does not fit (or whose pruning walk overflows) is marked to
+// run the whole program.  `counters[0]` accumulates the pool entries asked
+// for, `counters[1]` the stack high-water mark over cells.
 struct PruneArgs
 {
 	const Instr* prog;
 	const Entry* entries;   // whole program as entries
 	uint32_t n_entries;
 	uint32_t n_binary;
-	float gx, gy, gz, dx, dy, dz;
-	uint32_t bx, by, bz;    // blocks per axis
+	float ox, oy, oz;       // cell grid origin
+	float cx, cy, cz;       // cell size per axis
+	float radius;           // validity radius around each cell centre
+	uint32_t nx, ny, nz;    // cells per axis
 	uint32_t slots;         // LDS spill slots (whole-program max stack - 1)
 	uint32_t whole_flops;   // cost of one whole-program evaluation
-	Entry* pool;            // block programs, `cap` entries per block
-	uint32_t cap;
-	BlockDesc* desc;
-	uint32_t* max_stack;    // atomic max over blocks
+	Entry* pool;
+	uint64_t capacity;      // pool entries
+	BlockDesc* desc;        // one per cell
+	unsigned long long* counters;
 };
 
 // Level 2 + evaluation: one workgroup per block.
@@ -108,19 +115,75 @@ struct BlockEvalArgs
 	int waves;                   // waves per workgroup
 };
 
+// ---- sphere tracing -------------------------------------------------------------
+
+constexpr int kTraceLevels = 4;
+
+struct TraceLevel
+{
+	const BlockDesc* desc;   // null: level absent
+	const Entry* pool;       // this level's pruned programs
+	uint32_t n;              // cells per axis (cubic cells over the domain)
+	float cell;              // cell edge
+};
+
+struct TraceArgs
+{
+	const Instr* prog;
+	const Entry* whole;
+	uint32_t n_whole;
+	uint32_t whole_flops;
+	TraceLevel levels[kTraceLevels];  // coarse to fine
+	int nlevels;
+	float dom[3];                     // domain origin
+	// rays: camera (width * height pixels) or arrays (nrays)
+	int camera;
+	float eye[3], fwd[3], right[3], up[3];
+	float kx, ky;
+	uint32_t width, height;
+	const float* starts;
+	const float* dirs;
+	uint32_t nrays;
+	int max_steps;
+	float eps, far_, h;               // h <= 0: no normals
+	float* t_out;
+	float* n_out;                     // 3 per ray, or null
+	float* p_out;                     // 3 per ray, or null
+	uint32_t* steps_out;              // or null
+	uint8_t* hit_out;                 // or null
+	unsigned long long* work;         // [0] evaluations, [1] cost units, [2] hits
+	uint32_t slots;
+};
+
+// Central-difference or tetrahedral gradients at points.
+struct GradArgs
+{
+	const Instr* prog;
+	const float* xyz;
+	float* out;
+	uint64_t count;
+	float h;
+	int tetra;                        // 1: SDFNode::Gradient's tetrahedral rule
+	uint32_t slots;
+};
+
 // Launchers, one set per numerics build (exact = reference bit parity,
 // fast = contracted FP32).  `p` is points per lane for the tile kernel (4 or 8).
 namespace exact {
 hipError_t launch_points(const PointArgs& a, hipStream_t s);
 hipError_t launch_tiles(const GridArgs& a, int p, hipStream_t s);
-hipError_t launch_prune_blocks(const PruneArgs& a, hipStream_t s);
+hipError_t launch_prune_cells(const PruneArgs& a, hipStream_t s);
 hipError_t launch_eval_blocks(const BlockEvalArgs& a, int p, hipStream_t s);
+hipError_t launch_trace(const TraceArgs& a, hipStream_t s);
+hipError_t launch_gradient(const GradArgs& a, hipStream_t s);
 }
 namespace fast {
 hipError_t launch_points(const PointArgs& a, hipStream_t s);
 hipError_t launch_tiles(const GridArgs& a, int p, hipStream_t s);
-hipError_t launch_prune_blocks(const PruneArgs& a, hipStream_t s);
+hipError_t launch_prune_cells(const PruneArgs& a, hipStream_t s);
 hipError_t launch_eval_blocks(const BlockEvalArgs& a, int p, hipStream_t s);
+hipError_t launch_trace(const TraceArgs& a, hipStream_t s);
+hipError_t launch_gradient(const GradArgs& a, hipStream_t s);
 }
 
 // LDS layout of the block kernel (bytes), shared by launcher and kernel.

@@ -156,6 +156,25 @@ class SDF:
     def eval_points(self, points, fast: bool = False) -> np.ndarray:
         return self.program().eval_points(points, fast=fast)
 
+    def gradient(self, *args, fast: bool = False) -> np.ndarray:
+        """Lua `gradient(p)` (lua_sdf.cpp:398-407): SDFNode::Gradient's tetrahedral rule."""
+        from .trace import gradient
+        return gradient(self.program(), np.array([_vec3(args)], dtype=np.float32), h=1e-4, fast=fast)[0]
+
+    def ray_cast(self, origin, direction, max_iterations: int = 100, epsilon: float = 0.001, fast: bool = False):
+        """Lua `ray_cast(origin, dir, [iters], [eps])` (lua_sdf.cpp:410-448): hit position or None."""
+        from .trace import raymarch
+        hit, _, pos = raymarch(self.program(), [_vec3((origin,))], [_vec3((direction,))], max_iterations, epsilon,
+                               fast=fast)
+        return pos[0] if hit[0] else None
+
+    def magnet(self, origin, target, max_iterations: int = 100, epsilon: float = 0.001, fast: bool = False):
+        """Lua `magnet(origin, target, ...)`: ray_cast toward normalize(target - origin)."""
+        o = np.array(_vec3((origin,)), dtype=np.float32)
+        d = np.array(_vec3((target,)), dtype=np.float32) - o
+        d = d * (np.float32(1.0) / np.sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]))
+        return self.ray_cast(o, d, max_iterations, epsilon, fast)
+
 
 def _info_dict(info) -> dict:
     return {

@@ -1,0 +1,128 @@
+"""GPU sphere tracing and gradients.
+
+Bars:
+  * exact numerics: hit flags, travel, step counts and normals bit-identical
+    to the oracle's interpreter-based march / gradients on the same rays;
+  * against the reference tree walk (golden RayMarch / Gradient outputs):
+    identical hit flags except rays whose closest approach lies within a few
+    ulps of eps, |t - t_ref| <= 1e-5 * scene extent * steps, gradient
+    directions within 2e-2 (the tetrahedral step 1e-4 amplifies the
+    interpreter/tree-walk rounding difference by 1/h);
+  * fast numerics: |t| within 1e-4 * extent of exact on rays that hit in both.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from conftest import golden_spec, load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+
+
+def unit(d):
+    d = np.asarray(d, dtype=np.float32)
+    inv = np.float32(1.0) / np.sqrt((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2])
+    return (d * inv[:, None]).astype(np.float32)
+
+
+@pytest.mark.parametrize("name,extent", [("points_config1.npz", 5.0), ("points_rand16_s5.npz", 12.0),
+                                         ("points_rand256_s1.npz", 12.0)])
+def test_raymarch_matches_oracle_and_reference(gpu, oracle, name, extent):
+    from tangerine_amd.sdf import Program
+    from tangerine_amd.trace import raymarch
+    g = load_golden(name)
+    prog = Program.from_words(g["words"])
+    hit, t, pos = raymarch(prog, g["ray_start"], g["ray_dir"], 100, 1e-3)
+    # oracle interpreter march on the glm-normalised directions, no far cutoff
+    oh, ot, _ = oracle.raymarch_far(g["words"], g["ray_start"], unit(g["ray_dir"]), 100, 1e-3, math.inf)
+    assert np.array_equal(hit, oh)
+    assert np.array_equal(bits(t), bits(ot))
+    # reference tree walk
+    ref_hit = g["ray_hit"]
+    agree = hit == ref_hit
+    assert agree.mean() >= 0.99
+    both = hit & ref_hit
+    assert np.all(np.abs(t[both] - g["ray_travel"][both]) <= 1e-5 * extent * 100)
+    assert np.all(np.abs(pos[both] - g["ray_pos"][both]) <= 1e-5 * extent * 100)
+
+
+@pytest.mark.parametrize("name", ["points_config1.npz", "points_rand16_s5.npz", "points_rand256_s1.npz"])
+def test_gradients(gpu, oracle, name):
+    from tangerine_amd.sdf import Program
+    from tangerine_amd.trace import gradient
+    g = load_golden(name)
+    prog = Program.from_words(g["words"])
+    tet = gradient(prog, g["grad_points"], h=1e-4)
+    assert np.array_equal(bits(tet), bits(oracle.gradient_tetra(g["words"], g["grad_points"], 1e-4)))
+    ok = np.isfinite(g["gradient"]).all(axis=1)
+    cosang = np.sum(tet[ok] * g["gradient"][ok], axis=1)
+    assert np.quantile(cosang, 0.01) > 1 - 2e-2
+    cen = gradient(prog, g["grad_points"], h=1e-3, central=True)
+    assert np.array_equal(bits(cen), bits(oracle.gradient_central(g["words"], g["grad_points"], 1e-3)))
+
+
+def _render_case(oracle, spec, eye, width, height, prune, fast=False):
+    from tangerine_amd import scenes, trace
+    node = scenes.build_sdf(spec)
+    prog = node.program()
+    cam = trace.look_at(eye, (0, 0, 0), (0, 0, 1), 60.0, width, height)
+    tr = trace.Tracer()
+    st = tr.render(prog, cam, 256, 1e-4, 40.0, 1e-3, fast=fast, prune=prune)
+    t, n, steps = tr.image()
+    tr.close()
+    return node, prog, cam, st, t, n, steps
+
+
+@pytest.mark.parametrize("prune", [True, False])
+def test_render_bit_identical_to_oracle(gpu, oracle, prune):
+    """A 160x90 render of the 256-primitive tree from a seeded eye on the
+    radius-12 sphere: every pixel's t, step count and normal."""
+    from tangerine_amd import scenes, trace
+    spec = scenes.random_csg_spec(1, 256)
+    eye = trace.seeded_eye(3)
+    node, prog, cam, st, t, n, steps = _render_case(oracle, spec, eye, 160, 90, prune)
+    words = node.compile()
+    dirs = trace.camera_directions(cam)
+    starts = np.repeat(np.array([cam.eye[:]], dtype=np.float32), dirs.shape[0], axis=0)
+    oh, ot, osteps = oracle.raymarch_far(words, starts, dirs, 256, 1e-4, 40.0)
+    hits = np.isfinite(t)
+    assert np.array_equal(hits, oh)
+    assert 0.05 < hits.mean() < 0.95
+    assert np.array_equal(bits(t), bits(ot))
+    assert np.array_equal(steps.astype(np.int32), osteps)
+    # normals at the hit positions p = dir * t + eye (same float order as the march)
+    p = (dirs[hits] * t[hits][:, None] + starts[hits]).astype(np.float32)
+    on = oracle.gradient_central(words, p, 1e-3)
+    assert np.array_equal(bits(n[hits]), bits(on))
+    assert np.all(n[~hits] == 0)
+    assert st["hits"] == hits.sum() and st["rays"] == 160 * 90
+
+
+def test_render_fast_close_to_exact(gpu, oracle):
+    from tangerine_amd import scenes, trace
+    spec = scenes.random_csg_spec(1, 256)
+    eye = trace.seeded_eye(3)
+    _, _, _, _, te, ne, _ = _render_case(oracle, spec, eye, 160, 90, True, False)
+    _, _, _, _, tf, nf, _ = _render_case(oracle, spec, eye, 160, 90, True, True)
+    both = np.isfinite(te) & np.isfinite(tf)
+    assert (np.isfinite(te) == np.isfinite(tf)).mean() > 0.995
+    assert np.quantile(np.abs(te[both] - tf[both]), 0.999) <= 1e-4 * 12.0
+    assert np.quantile(np.sum(ne[both] * nf[both], axis=1), 0.01) > 0.99
+
+
+def test_lua_queries(gpu):
+    from tangerine_amd import sdf
+    model = sdf.blend_union(sdf.sphere(2), sdf.box(1.6, 1, 1).move_x(0.6), 0.2).diff(sdf.cylinder(0.6, 3))
+    p = model.ray_cast((5, 0.5, 0.1), (-1, 0, 0))
+    assert p is not None and abs(model.eval(p)) <= 1e-3
+    assert model.ray_cast((5, 5, 5), (1, 1, 1)) is None
+    q = model.magnet((0, 5, 0.2), (0, 0, 0.2))
+    assert q is not None and q[1] > 0
+    gdir = model.gradient(0, 2, 0)
+    np.testing.assert_allclose(gdir, [0, 1, 0], atol=1e-2)
