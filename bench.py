@@ -101,6 +101,67 @@ def cpu_baseline(spec, lo, hi, cells, budget_s: float, threads: int) -> dict:
     }
 
 
+def surrogate_leg(mesher, prog, wl, origin, step, cells, args) -> dict:
+    """Fit the brick-culling surrogate, time culled exports of the same grid,
+    and judge them against the plain export that ran last."""
+    import hashlib
+    from tangerine_amd.surrogate import Surrogate
+
+    def digest(m):
+        v, t = m.mesh()
+        return hashlib.sha256(v.tobytes()).hexdigest(), hashlib.sha256(t.tobytes()).hexdigest(), v.shape, t.shape
+
+    plain_digest = digest(mesher)
+    active = mesher.active_bricks().astype(bool)
+    pad = 0.25
+    lo = (wl["lo"] - pad,) * 3
+    sur = Surrogate(lo, (wl["hi"] - wl["lo"]) + 2 * pad, sigma=2.0, clamp=args.sur_clamp, seed=wl["seed"])
+    t0 = time.perf_counter()
+    fit = sur.fit(prog, samples=args.sur_samples, calib_samples=200_000, surface_sigma=0.05, epochs=args.sur_epochs,
+                  fast=args.fast)
+    fit_wall = time.perf_counter() - t0
+    loss = sur.loss_log()
+    log(f"surrogate fit: {fit_wall:.2f} s, margin {fit['margin']:.4f}, mean residual {fit['mean_residual']:.4f}, "
+        f"loss {loss[0]:.4g} -> {loss[-1]:.4g}")
+
+    def culled_once():
+        return mesher.export_culled(prog, sur, origin, step, cells, fast=args.fast, prune=not args.no_prune)
+
+    for i in range(args.warmup):
+        st = culled_once()
+        log(f"culled warmup {i}: predict {st['predict_ms']:.1f} eval {st['eval_ms']:.1f} mesh {st['mesh_ms']:.1f} ms")
+    totals, evals_ms = [], []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        st = culled_once()
+        totals.append(st["total_ms"])
+        evals_ms.append(st["eval_ms"])
+        log(f"culled step {i}: predict {st['predict_ms']:.1f} eval {st['eval_ms']:.1f} mesh {st['mesh_ms']:.1f} "
+            f"total {st['total_ms']:.1f} ms, kept {st['bricks_flagged']}/{st['bricks_total']} bricks")
+    _lib.check(_lib.load().tg_device_synchronize(), "sync")
+    wall = time.perf_counter() - t0
+    kept = mesher.brick_flags().astype(bool)
+    missed = int(np.count_nonzero(active & ~kept))
+    same = digest(mesher) == plain_digest
+    sur.close()
+    return {
+        "export_ms_culled": 1000.0 * wall / args.steps,
+        "export_ms_culled_device": float(np.mean(totals)),
+        "eval_ms_culled": float(np.mean(evals_ms)),
+        "identical_mesh": bool(same),
+        "missed_bricks": missed,
+        "bricks_kept": int(st["bricks_flagged"]),
+        "bricks_total": int(st["bricks_total"]),
+        "bricks_active": int(active.sum()),
+        "exact_evals_culled": int(st["exact_evals"]),
+        "fit": {"wall_ms": 1000.0 * fit_wall, "train_ms": fit["train_ms"], "sample_ms": fit["sample_ms"],
+                "samples": fit["samples"], "calib_samples": fit["calib_samples"], "steps": fit["steps"],
+                "margin": fit["margin"], "mean_residual": fit["mean_residual"],
+                "loss_first": float(loss[0]), "loss_last": float(loss[-1]), "clamp": args.sur_clamp,
+                "domain": [lo[0], lo[0] + (wl["hi"] - wl["lo"]) + 2 * pad]},
+    }
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +171,10 @@ def main(argv=None) -> int:
     ap.add_argument("--fast", action="store_true", help="contracted FP32 numerics instead of bit parity")
     ap.add_argument("--no-prune", action="store_true", help="interpret the whole program in every tile")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate-culled export leg")
+    ap.add_argument("--sur-samples", type=int, default=2 * 1024 * 1024)
+    ap.add_argument("--sur-epochs", type=int, default=16)
+    ap.add_argument("--sur-clamp", type=float, default=0.5)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("TG_CPU_THREADS", "16")))
     args = ap.parse_args(argv)
@@ -153,6 +218,9 @@ def main(argv=None) -> int:
     _lib.check(lib.tg_device_synchronize(), "sync")
     wall = time.perf_counter() - t0
     nverts, ntris = mesher.counts()
+    culled = None
+    if not args.no_surrogate:
+        culled = surrogate_leg(mesher, prog, wl, origin, step, cells, args)
     mesher.close()
 
     ms_per_step = 1000.0 * wall / args.steps
@@ -183,6 +251,7 @@ def main(argv=None) -> int:
         "mesh_ms": float(np.mean(mesh_ms)),
         "samples_per_s": samples / (ms_per_step * 1e-3),
         "mesh": {"vertices": nverts, "triangles": ntris},
+        "surrogate_culled": culled,
         "roofline": {
             "bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,

@@ -58,6 +58,7 @@ typedef struct tg_node tg_node;
 typedef struct tg_program tg_program;
 typedef struct tg_mesher tg_mesher;
 typedef struct tg_tracer tg_tracer;
+typedef struct tg_surrogate tg_surrogate;
 
 /* Pinhole camera: pixel (i, j) looks along
  *   normalize(forward + sx*tan_half_x*right + sy*tan_half_y*up),
@@ -111,7 +112,24 @@ typedef struct tg_export_stats
 	uint64_t tiles_evaluated;  /* 8^3-sample tiles interpreted */
 	uint64_t exact_evals;      /* samples actually interpreted */
 	double flops;              /* algorithmic FP32 work executed (cost table) */
+	uint64_t bricks_total;     /* culled export: 8^3-cube bricks of the grid (0 otherwise) */
+	uint64_t bricks_flagged;   /* culled export: bricks the surrogate kept for exact evaluation */
+	double predict_ms;         /* culled export: surrogate prediction and tile selection (HIP events) */
 } tg_export_stats;
+
+typedef struct tg_surrogate_stats
+{
+	double sample_ms;          /* coarse mesh, sample drawing and exact labels (host clock) */
+	double train_ms;           /* replayed training graphs (HIP events) */
+	double total_ms;           /* whole fit (host clock) */
+	uint64_t samples;          /* training samples */
+	uint64_t calib_samples;    /* held-out calibration samples */
+	uint64_t surface_vertices; /* vertices of the coarse exact mesh the near-surface samples start from */
+	uint32_t steps;            /* Adam steps */
+	float final_loss;          /* mean squared error of the last batch */
+	float margin;              /* max |prediction - clamped exact distance| over the calibration samples */
+	float mean_residual;       /* mean of the same */
+} tg_surrogate_stats;
 
 /* ---- errors / device ----------------------------------------------------- */
 const char* tg_last_error(void);
@@ -199,7 +217,8 @@ int tg_tracer_set_domain(tg_tracer* tracer, const float lo[3], float size);
 int tg_tracer_render(tg_tracer* tracer, const tg_program* program, const tg_camera* camera, int max_steps,
                      float epsilon, float far_distance, float normal_h, uint32_t flags);
 /* t: 1 float per pixel (INFINITY = miss); normals: 3 floats per pixel or NULL;
- * steps: 1 uint32 per pixel or NULL.  Row-major, row 0 at the top. */
+ * steps: distance evaluations along each pixel's ray (the final hit test
+ * included), 1 uint32 per pixel, or NULL.  Row-major, row 0 at the top. */
 int tg_tracer_copy(const tg_tracer* tracer, float* t, float* normals, uint32_t* steps);
 int tg_tracer_stats(const tg_tracer* tracer, tg_trace_stats* out);
 
@@ -215,6 +234,48 @@ int tg_mesher_copy_mesh(const tg_mesher* mesher, float* vertices, uint32_t* tria
 /* Samples of the last export, x fastest. */
 int tg_mesher_copy_grid(const tg_mesher* mesher, float* values);
 int tg_mesher_stats(const tg_mesher* mesher, tg_export_stats* out);
+
+/* ---- learned brick culling (no reference counterpart) ----------------------
+ * A Fourier-feature MLP (3 -> 96 features, sigma-scaled Gaussian frequencies
+ * -> 64 SiLU -> 64 SiLU -> 1; BF16 matrix-core operands, FP32 accumulation)
+ * fitted to clamp(d, -clamp, clamp) over the cube [lo, lo + size]^3.  Its only
+ * use is to choose which 8^3-cube bricks a culled export evaluates exactly:
+ * a brick is skipped when |prediction at its centre| > margin + its
+ * half-diagonal, where margin is the largest residual seen on held-out
+ * calibration samples.  Every sample and vertex of the output still comes
+ * from the exact evaluator; a brick holding surface that the surrogate
+ * wrongly skipped would be missing from the mesh, which
+ * tg_mesher_active_bricks lets a caller audit against a plain export. */
+int tg_surrogate_create(tg_surrogate** out, const float lo[3], float size, float sigma, float clamp, uint64_t seed);
+void tg_surrogate_destroy(tg_surrogate* surrogate);
+/* Read (write = 0) or replace (write = 1) the FP32 parameters (10433 floats:
+ * W1[64][96], b1[64], W2[64][64], b2[64], w3[64], b3) and the frequencies
+ * (48 x 3 floats); either pointer may be NULL. */
+int tg_surrogate_params(tg_surrogate* surrogate, float* params, float* fourier, int write);
+int tg_surrogate_predict(const tg_surrogate* surrogate, const float* xyz, uint64_t count, float* out);
+/* Gradient of the mean squared error against clamp(target) over count
+ * points (count a positive multiple of 32) and the loss itself. */
+int tg_surrogate_gradients(tg_surrogate* surrogate, const float* xyz, const float* target, uint64_t count,
+                           float* grad, float* loss);
+/* Draw `samples` training and `calib_samples` calibration points on the
+ * device (half uniform, half near the surface of a 128^3 exact export of the
+ * domain with N(0, surface_sigma^2) offsets), label them with `exact`, run
+ * `epochs` passes of Adam (batches of 16384) and calibrate the margin. */
+int tg_surrogate_fit(tg_surrogate* surrogate, const tg_program* exact, uint64_t samples, uint64_t calib_samples,
+                     float surface_sigma, uint32_t epochs, uint32_t flags, tg_surrogate_stats* stats);
+int tg_surrogate_set_margin(tg_surrogate* surrogate, float margin);
+int tg_surrogate_get_stats(const tg_surrogate* surrogate, tg_surrogate_stats* out);
+/* Mean training loss per Adam step of the last fit. */
+int tg_surrogate_loss_log(const tg_surrogate* surrogate, float* out, uint64_t capacity, uint64_t* count);
+/* Export that evaluates exactly only the tiles of bricks the surrogate keeps.
+ * Needs a 1-Lipschitz program and a calibrated surrogate.  The samples of
+ * skipped tiles are not computed, so tg_mesher_copy_grid refuses afterwards. */
+int tg_mesher_export_culled(tg_mesher* mesher, const tg_program* program, const tg_grid* grid,
+                            const tg_surrogate* surrogate, uint32_t flags);
+/* One byte per 8^3-cube brick (x fastest): the kept bricks of the last culled
+ * export, or the bricks holding an active cube of the last export. */
+int tg_mesher_brick_flags(const tg_mesher* mesher, uint8_t* out, uint64_t capacity, uint64_t* count);
+int tg_mesher_active_bricks(tg_mesher* mesher, uint8_t* out, uint64_t capacity, uint64_t* count);
 
 #ifdef __cplusplus
 }

@@ -987,7 +987,7 @@ void tgo_raymarch(void* a, const float* start, const float* dir, uint64_t n, int
 // Sphere tracing over the interpreter with a far cutoff (no reference
 // counterpart for the cutoff): the direction is used as given (callers pass
 // unit vectors); hit when d <= eps, miss when t > far or steps run out.
-// Returns the step count per ray in `steps`.
+// Returns per ray in `steps` the distance evaluations made, the hit test included.
 int tgo_raymarch_far_words(const uint32_t* words, uint64_t nwords, const float* start, const float* dir, uint64_t n,
 	int iters, float eps, float far_, uint8_t* hit, float* travel, int32_t* steps)
 {
@@ -998,12 +998,13 @@ int tgo_raymarch_far_words(const uint32_t* words, uint64_t nwords, const float* 
 		V3 ps = s;
 		float t = 0.0f;
 		bool h = false;
-		int it = 0;
-		for (; it < iters; ++it)
+		int evals = 0;  // distance evaluations along the ray, the hit test included
+		for (int it = 0; it < iters; ++it)
 		{
 			bool ok;
 			float dist = interpret(words, nwords, ps, &ok);
 			if (!ok) return -1;
+			++evals;
 			if (dist <= eps) { h = true; break; }
 			t += dist;
 			if (t > far_) break;
@@ -1011,7 +1012,7 @@ int tgo_raymarch_far_words(const uint32_t* words, uint64_t nwords, const float* 
 		}
 		hit[i] = h;
 		travel[i] = h ? t : std::numeric_limits<float>::infinity();
-		steps[i] = it;
+		steps[i] = evals;
 	}
 	return 0;
 }

@@ -38,7 +38,14 @@ class TgProgramInfo(C.Structure):
 
 class TgExportStats(C.Structure):
     _fields_ = [("eval_ms", C.c_double), ("mesh_ms", C.c_double), ("total_ms", C.c_double),
-                ("samples", _u64), ("tiles_evaluated", _u64), ("exact_evals", _u64), ("flops", C.c_double)]
+                ("samples", _u64), ("tiles_evaluated", _u64), ("exact_evals", _u64), ("flops", C.c_double),
+                ("bricks_total", _u64), ("bricks_flagged", _u64), ("predict_ms", C.c_double)]
+
+
+class TgSurrogateStats(C.Structure):
+    _fields_ = [("sample_ms", C.c_double), ("train_ms", C.c_double), ("total_ms", C.c_double),
+                ("samples", _u64), ("calib_samples", _u64), ("surface_vertices", _u64), ("steps", _u32),
+                ("final_loss", _f), ("margin", _f), ("mean_residual", _f)]
 
 
 class TgCamera(C.Structure):
@@ -108,6 +115,18 @@ SIGNATURES = {
     "tg_mesher_copy_mesh": (C.c_int, [_p, _fp, _u32p]),
     "tg_mesher_copy_grid": (C.c_int, [_p, _fp]),
     "tg_mesher_stats": (C.c_int, [_p, C.POINTER(TgExportStats)]),
+    "tg_mesher_export_culled": (C.c_int, [_p, _p, C.POINTER(TgGrid), _p, _u32]),
+    "tg_mesher_brick_flags": (C.c_int, [_p, C.POINTER(C.c_uint8), _u64, _u64p]),
+    "tg_mesher_active_bricks": (C.c_int, [_p, C.POINTER(C.c_uint8), _u64, _u64p]),
+    "tg_surrogate_create": (C.c_int, [C.POINTER(_p), _fp, _f, _f, _f, _u64]),
+    "tg_surrogate_destroy": (None, [_p]),
+    "tg_surrogate_params": (C.c_int, [_p, _fp, _fp, C.c_int]),
+    "tg_surrogate_predict": (C.c_int, [_p, _fp, _u64, _fp]),
+    "tg_surrogate_gradients": (C.c_int, [_p, _fp, _fp, _u64, _fp, _fp]),
+    "tg_surrogate_fit": (C.c_int, [_p, _p, _u64, _u64, _f, _u32, _u32, C.POINTER(TgSurrogateStats)]),
+    "tg_surrogate_set_margin": (C.c_int, [_p, _f]),
+    "tg_surrogate_get_stats": (C.c_int, [_p, C.POINTER(TgSurrogateStats)]),
+    "tg_surrogate_loss_log": (C.c_int, [_p, _fp, _u64, _u64p]),
 }
 
 

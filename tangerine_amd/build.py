@@ -43,6 +43,8 @@ HIP_UNITS = [
     ("tg_export.hip", ["-ffp-contract=off"]),
     ("tg_prune.hip", ["-ffp-contract=off"]),
     ("tg_trace.hip", ["-ffp-contract=off"]),
+    ("tg_mlp.hip", ["-ffp-contract=off"]),
+    ("tg_surrogate.hip", ["-ffp-contract=off"]),
 ]
 
 HOST_UNITS = ["tg_tree.cpp", "tg_program.cpp"]

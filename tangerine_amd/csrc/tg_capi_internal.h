@@ -56,3 +56,14 @@ struct CellPrograms
 int prune_cells(hipStream_t stream, const tg_program* p, const float origin[3], const float cell[3], float radius,
 	const uint32_t counts[3], bool fast, CellPrograms& out);
 }
+
+struct tg_mesher;
+struct tg_surrogate;
+#include "tg_mlp.h"
+
+namespace tg {
+// Device vertices of the mesher's last export (3 floats each).
+const float* mesher_vertices(const tg_mesher* m);
+mlp::Net surrogate_net(const tg_surrogate* s);
+float surrogate_margin(const tg_surrogate* s);
+}

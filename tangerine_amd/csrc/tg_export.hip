@@ -14,6 +14,7 @@
 #include "tg_capi_internal.h"
 #include "tg_kernels.h"
 #include "tg_mesh.h"
+#include "tg_mlp.h"
 #include "tg_types.h"
 
 using namespace tg;
@@ -31,6 +32,8 @@ struct tg_mesher
 	hipEvent_t ev[4] = {};
 	DevBuf pool, signs, active, word_count, word_base, quad_count, quad_base, vertices, triangles, scan_tmp, dense;
 	DevBuf counters;
+	DevBuf brick_flags, brick_active, tile_mask, block_list, tile_list;
+	uint64_t nbricks = 0;         // bricks of the last culled export (0: none)
 	CellPrograms blocks;
 	MeshArgs args{};
 	uint64_t nverts = 0, ntris = 0;
@@ -134,6 +137,71 @@ int surface_nets(tg_mesher* m)
 
 constexpr size_t kMaxLds = 160 * 1024;
 
+// Thread per cube brick: a kept brick (cubes [8b, 8b + 8) per axis, corner
+// samples [8b, 8b + 8]) needs sample tiles b and b + 1 on each axis.  Sets
+// their bits in the per-block tile masks (bit t = x + 4 y + 16 z inside a
+// block of 4^3 tiles) and counts kept bricks (counter[0]).
+__global__ void __launch_bounds__(256) k_tiles_from_bricks(const uint8_t* flags, uint32_t cbx, uint32_t cby,
+	uint32_t cbz, uint32_t tx, uint32_t ty, uint32_t tz, uint32_t bx, uint32_t by, unsigned long long* mask,
+	unsigned long long* counter)
+{
+	const uint64_t b = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	const uint64_t n = uint64_t(cbx) * cby * cbz;
+	const bool keep = b < n && flags[b] != 0;
+	const uint64_t votes = __ballot(keep);
+	if ((threadIdx.x & 63) == 0 && votes) atomicAdd(counter, (unsigned long long)__popcll(votes));
+	if (!keep) return;
+	const uint32_t x = uint32_t(b % cbx), y = uint32_t((b / cbx) % cby), z = uint32_t(b / (uint64_t(cbx) * cby));
+	for (uint32_t dz = 0; dz < 2; ++dz)
+		for (uint32_t dy = 0; dy < 2; ++dy)
+			for (uint32_t dx = 0; dx < 2; ++dx)
+			{
+				const uint32_t ix = x + dx, iy = y + dy, iz = z + dz;
+				if (ix >= tx || iy >= ty || iz >= tz) continue;
+				const uint32_t blk = (ix >> 2) + bx * ((iy >> 2) + by * (iz >> 2));
+				const uint32_t bit = (ix & 3) + 4 * (iy & 3) + 16 * (iz & 3);
+				atomicOr(&mask[blk], 1ull << bit);
+			}
+}
+
+// Thread per block: list the blocks with at least one tile to evaluate.
+__global__ void __launch_bounds__(256) k_list_blocks(const unsigned long long* mask, uint32_t nblocks, uint32_t* list,
+	unsigned long long* counter)
+{
+	const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+	if (b >= nblocks || mask[b] == 0) return;
+	list[atomicAdd(counter, 1ull)] = b;
+}
+
+// Thread per tile: list the tiles whose mask bit is set.
+__global__ void __launch_bounds__(256) k_list_tiles(const unsigned long long* mask, uint32_t tx, uint32_t ty,
+	uint32_t tz, uint32_t bx, uint32_t by, uint32_t* list, unsigned long long* counter)
+{
+	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (t >= uint64_t(tx) * ty * tz) return;
+	const uint32_t ix = uint32_t(t % tx), iy = uint32_t((t / tx) % ty), iz = uint32_t(t / (uint64_t(tx) * ty));
+	const uint32_t blk = (ix >> 2) + bx * ((iy >> 2) + by * (iz >> 2));
+	const uint32_t bit = (ix & 3) + 4 * (iy & 3) + 16 * (iz & 3);
+	if ((mask[blk] >> bit) & 1ull) list[atomicAdd(counter, 1ull)] = uint32_t(t);
+}
+
+// Thread per active-cube word: mark the bricks holding an active cube.
+__global__ void __launch_bounds__(256) k_active_bricks(const uint64_t* active, uint32_t aw, uint32_t sy, uint32_t sz,
+	uint32_t cbx, uint32_t cby, uint8_t* out)
+{
+	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (t >= uint64_t(sy) * sz * aw) return;
+	const uint64_t act = active[t];
+	if (!act) return;
+	const uint32_t w = uint32_t(t % aw);
+	const uint64_t row = t / aw;
+	const uint32_t j = uint32_t(row % sy), k = uint32_t(row / sy);
+	for (int b = 0; b < 8; ++b)
+		if ((act >> (8 * b)) & 0xffull) out[(w * 8 + b) + uint64_t(cbx) * ((j >> 3) + uint64_t(cby) * (k >> 3))] = 1;
+}
+
+unsigned grid_for(uint64_t n) { return unsigned((n + 255) / 256); }
+
 // Two-level pruned evaluation of every tile (k_prune_cells over 32^3-sample
 // blocks, then k_eval_blocks).  Returns 1 if it ran, 0 if the program is not
 // prunable (the caller then interprets the whole program per tile), <0 on
@@ -219,7 +287,8 @@ extern "C" void tg_mesher_destroy(tg_mesher* m)
 	if (!m) return;
 	if (m->stream) hipStreamSynchronize(m->stream);
 	for (DevBuf* b : { &m->pool, &m->signs, &m->active, &m->word_count, &m->word_base, &m->quad_count, &m->quad_base,
-	                   &m->vertices, &m->triangles, &m->scan_tmp, &m->dense, &m->counters })
+	                   &m->vertices, &m->triangles, &m->scan_tmp, &m->dense, &m->counters, &m->brick_flags,
+	                   &m->brick_active, &m->tile_mask, &m->block_list, &m->tile_list })
 		b->release();
 	m->blocks.release();
 	for (hipEvent_t e : m->ev)
@@ -228,12 +297,78 @@ extern "C" void tg_mesher_destroy(tg_mesher* m)
 	delete m;
 }
 
-extern "C" int tg_mesher_export(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t flags)
+namespace {
+
+// Surrogate stage of a culled export: brick flags, per-block tile masks and
+// the list of blocks (or, for the unpruned evaluator, tiles) to evaluate.
+int select_tiles(tg_mesher* m, const tg_grid* g, const tg_surrogate* s, bool tiles, uint32_t* listed,
+	uint64_t* flagged)
 {
-	if (!m || !p) return fail(TG_ERR_INVALID, "tg_mesher_export: null argument");
+	MeshArgs& ma = m->args;
+	const uint32_t bx = (ma.tx + kBlockTiles - 1) / kBlockTiles, by = (ma.ty + kBlockTiles - 1) / kBlockTiles,
+	               bz = (ma.tz + kBlockTiles - 1) / kBlockTiles;
+	const uint64_t nblocks = uint64_t(bx) * by * bz;
+	const uint64_t ntiles = uint64_t(ma.tx) * ma.ty * ma.tz;
+	const uint64_t nbricks = uint64_t(ma.cbx) * ma.cby * ma.cbz;
+	int rc = m->brick_flags.ensure(nbricks);
+	if (!rc) rc = m->tile_mask.ensure(nblocks * 8);
+	if (!rc) rc = tiles ? m->tile_list.ensure(ntiles * 4) : m->block_list.ensure(nblocks * 4);
+	if (!rc) rc = m->counters.ensure(64);
+	if (rc) return rc;
+	unsigned long long* ctr = m->counters.as<unsigned long long>();
+	TG_CHECK_HIP(hipMemsetAsync(m->counters.ptr, 0, 64, m->stream), "counter reset");
+	TG_CHECK_HIP(hipMemsetAsync(m->tile_mask.ptr, 0, nblocks * 8, m->stream), "mask reset");
+	mlp::PredictArgs pa{};
+	pa.net = surrogate_net(s);
+	pa.xyz = nullptr;
+	pa.count = nbricks;
+	pa.out = nullptr;
+	pa.gx = g->x; pa.gy = g->y; pa.gz = g->z;
+	pa.dx = g->dx; pa.dy = g->dy; pa.dz = g->dz;
+	pa.cbx = ma.cbx; pa.cby = ma.cby; pa.cbz = ma.cbz;
+	pa.radius = 4.0f * std::sqrt(g->dx * g->dx + g->dy * g->dy + g->dz * g->dz);
+	pa.threshold = surrogate_margin(s) + pa.radius;
+	pa.flags = m->brick_flags.as<uint8_t>();
+	if (!(pa.threshold < pa.net.clamp))
+	{
+		// The clamped fit cannot certify a brick this large: keep them all.
+		TG_CHECK_HIP(hipMemsetAsync(m->brick_flags.ptr, 1, nbricks, m->stream), "flag fill");
+	}
+	else
+		TG_CHECK_HIP(mlp::launch_predict(pa, m->stream), "brick prediction launch");
+	hipLaunchKernelGGL(k_tiles_from_bricks, dim3(grid_for(nbricks)), dim3(256), 0, m->stream,
+		m->brick_flags.as<const uint8_t>(), ma.cbx, ma.cby, ma.cbz, ma.tx, ma.ty, ma.tz, bx, by,
+		m->tile_mask.as<unsigned long long>(), ctr);
+	TG_CHECK_HIP(hipGetLastError(), "tile selection launch");
+	if (tiles)
+		hipLaunchKernelGGL(k_list_tiles, dim3(grid_for(ntiles)), dim3(256), 0, m->stream,
+			m->tile_mask.as<const unsigned long long>(), ma.tx, ma.ty, ma.tz, bx, by, m->tile_list.as<uint32_t>(), ctr + 1);
+	else
+		hipLaunchKernelGGL(k_list_blocks, dim3(grid_for(nblocks)), dim3(256), 0, m->stream,
+			m->tile_mask.as<const unsigned long long>(), uint32_t(nblocks), m->block_list.as<uint32_t>(), ctr + 1);
+	TG_CHECK_HIP(hipGetLastError(), "list launch");
+	unsigned long long c[2] = { 0, 0 };
+	TG_CHECK_HIP(hipMemcpyAsync(c, ctr, 16, hipMemcpyDeviceToHost, m->stream), "count read");
+	TG_CHECK_HIP(hipStreamSynchronize(m->stream), "tile selection");
+	*flagged = c[0];
+	*listed = uint32_t(c[1]);
+	m->nbricks = nbricks;
+	ma.brick_flags = m->brick_flags.as<const uint8_t>();
+	return TG_OK;
+}
+
+int run_export(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t flags, const tg_surrogate* sur)
+{
 	int rc = validate_grid(g);
 	if (rc) return rc;
+	if (sur)
+	{
+		if (!p->decoded.lipschitz)
+			return fail(TG_ERR_PROGRAM, "culled export needs a 1-Lipschitz program (no ellipsoid, no stretching scale)");
+		if (!(surrogate_margin(sur) >= 0.0f)) return fail(TG_ERR_INVALID, "culled export: surrogate is not calibrated");
+	}
 	m->have_grid = false;
+	m->nbricks = 0;
 	rc = layout(m, g);
 	if (rc) return rc;
 	MeshArgs& ma = m->args;
@@ -251,17 +386,39 @@ extern "C" int tg_mesher_export(tg_mesher* m, const tg_program* p, const tg_grid
 	ga.row_bytes = ma.row_bytes;
 	ga.slots = std::max<uint32_t>(1, p->decoded.max_stack - 1);
 
+	const bool try_blocks = !(flags & TG_FLAG_NO_PRUNE) && p->decoded.lipschitz;
 	TG_CHECK_HIP(hipEventRecord(m->ev[0], m->stream), "event");
-	uint64_t evals = uint64_t(ga.ntiles) * kTileSamples;
-	double flops = double(evals) * p->decoded.flops;
-	int pruned = 0;
-	if (!(flags & TG_FLAG_NO_PRUNE))
+	uint32_t listed = 0;
+	uint64_t flagged = 0;
+	if (sur)
 	{
-		pruned = eval_pruned(m, p, g, flags, nullptr, nullptr, 0, &evals, &flops);
+		rc = select_tiles(m, g, sur, !try_blocks, &listed, &flagged);
+		if (rc) return rc;
+	}
+	TG_CHECK_HIP(hipEventRecord(m->ev[3], m->stream), "event");
+	uint64_t evals = 0;
+	double flops = 0.0;
+	int pruned = 0;
+	if (try_blocks)
+	{
+		pruned = sur ? eval_pruned(m, p, g, flags, m->tile_mask.as<const uint64_t>(), m->block_list.as<const uint32_t>(),
+		                           listed, &evals, &flops)
+		             : eval_pruned(m, p, g, flags, nullptr, nullptr, 0, &evals, &flops);
 		if (pruned < 0) return pruned;
+		if (!pruned && sur)
+		{
+			// Block evaluator unavailable (LDS): list tiles for the plain one.
+			rc = select_tiles(m, g, sur, true, &listed, &flagged);
+			if (rc) return rc;
+		}
 	}
 	if (!pruned)
 	{
+		if (sur)
+		{
+			ga.tile_list = m->tile_list.as<const uint32_t>();
+			ga.ntiles = listed;
+		}
 		evals = uint64_t(ga.ntiles) * kTileSamples;
 		flops = double(evals) * p->decoded.flops;
 		hipError_t e = (flags & TG_FLAG_FAST) ? fast::launch_tiles(ga, 8, m->stream)
@@ -273,8 +430,9 @@ extern "C" int tg_mesher_export(tg_mesher* m, const tg_program* p, const tg_grid
 	if (rc) return rc;
 	TG_CHECK_HIP(hipEventRecord(m->ev[2], m->stream), "event");
 	TG_CHECK_HIP(hipEventSynchronize(m->ev[2]), "export");
-	float t_eval = 0, t_mesh = 0, t_all = 0;
-	hipEventElapsedTime(&t_eval, m->ev[0], m->ev[1]);
+	float t_pred = 0, t_eval = 0, t_mesh = 0, t_all = 0;
+	hipEventElapsedTime(&t_pred, m->ev[0], m->ev[3]);
+	hipEventElapsedTime(&t_eval, m->ev[3], m->ev[1]);
 	hipEventElapsedTime(&t_mesh, m->ev[1], m->ev[2]);
 	hipEventElapsedTime(&t_all, m->ev[0], m->ev[2]);
 	tg_export_stats& s = m->stats;
@@ -286,7 +444,61 @@ extern "C" int tg_mesher_export(tg_mesher* m, const tg_program* p, const tg_grid
 	s.tiles_evaluated = evals / kTileSamples;
 	s.exact_evals = evals;
 	s.flops = flops;
-	m->have_grid = true;
+	if (sur)
+	{
+		s.bricks_total = m->nbricks;
+		s.bricks_flagged = flagged;
+		s.predict_ms = t_pred;
+	}
+	m->have_grid = sur == nullptr;
+	return TG_OK;
+}
+
+}  // namespace
+
+const float* tg::mesher_vertices(const tg_mesher* m) { return m->vertices.as<const float>(); }
+
+extern "C" int tg_mesher_export(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t flags)
+{
+	if (!m || !p) return fail(TG_ERR_INVALID, "tg_mesher_export: null argument");
+	return run_export(m, p, g, flags, nullptr);
+}
+
+extern "C" int tg_mesher_export_culled(tg_mesher* m, const tg_program* p, const tg_grid* g, const tg_surrogate* s,
+	uint32_t flags)
+{
+	if (!m || !p || !s) return fail(TG_ERR_INVALID, "tg_mesher_export_culled: null argument");
+	return run_export(m, p, g, flags, s);
+}
+
+extern "C" int tg_mesher_brick_flags(const tg_mesher* m, uint8_t* out, uint64_t capacity, uint64_t* count)
+{
+	if (!m || !count) return fail(TG_ERR_INVALID, "tg_mesher_brick_flags: null argument");
+	*count = m->nbricks;
+	if (out && capacity && m->nbricks)
+		TG_CHECK_HIP(hipMemcpy(out, m->brick_flags.ptr, std::min(capacity, m->nbricks), hipMemcpyDeviceToHost),
+			"flag download");
+	return TG_OK;
+}
+
+extern "C" int tg_mesher_active_bricks(tg_mesher* m, uint8_t* out, uint64_t capacity, uint64_t* count)
+{
+	if (!m || !count) return fail(TG_ERR_INVALID, "tg_mesher_active_bricks: null argument");
+	const MeshArgs& a = m->args;
+	if (!a.active || a.sx == 0) return fail(TG_ERR_INVALID, "tg_mesher_active_bricks: nothing meshed yet");
+	const uint64_t n = uint64_t(a.cbx) * a.cby * a.cbz;
+	*count = n;
+	if (!out || !capacity) return TG_OK;
+	int rc = m->brick_active.ensure(n);
+	if (rc) return rc;
+	TG_CHECK_HIP(hipMemsetAsync(m->brick_active.ptr, 0, n, m->stream), "reset");
+	const uint64_t words = uint64_t(a.sy) * a.sz * a.aw;
+	hipLaunchKernelGGL(k_active_bricks, dim3(grid_for(words)), dim3(256), 0, m->stream, a.active, a.aw, a.sy, a.sz,
+		a.cbx, a.cby, m->brick_active.as<uint8_t>());
+	TG_CHECK_HIP(hipGetLastError(), "active brick launch");
+	TG_CHECK_HIP(hipMemcpyAsync(out, m->brick_active.ptr, std::min(capacity, n), hipMemcpyDeviceToHost, m->stream),
+		"download");
+	TG_CHECK_HIP(hipStreamSynchronize(m->stream), "active bricks");
 	return TG_OK;
 }
 
@@ -340,7 +552,8 @@ extern "C" int tg_mesher_copy_mesh(const tg_mesher* m, float* v, uint32_t* t)
 extern "C" int tg_mesher_copy_grid(const tg_mesher* m, float* values)
 {
 	if (!m || !values) return fail(TG_ERR_INVALID, "tg_mesher_copy_grid: null argument");
-	if (!m->have_grid) return fail(TG_ERR_INVALID, "tg_mesher_copy_grid: no grid evaluated yet");
+	if (!m->have_grid)
+		return fail(TG_ERR_INVALID, "tg_mesher_copy_grid: no fully evaluated grid (none yet, or the last export was culled)");
 	tg_mesher* mm = const_cast<tg_mesher*>(m);
 	const MeshArgs& a = mm->args;
 	const uint64_t plane = uint64_t(a.nx) * a.ny;

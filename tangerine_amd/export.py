@@ -41,6 +41,31 @@ class Mesher:
         self._cells = tuple(int(c) for c in cells)
         return self.stats()
 
+    def export_culled(self, program, surrogate, origin, step, cells, fast: bool = False, prune: bool = True) -> dict:
+        """Export that evaluates exactly only the bricks the surrogate keeps."""
+        g = _lib.grid_struct(origin, step, cells)
+        flags = (_lib.TG_FLAG_FAST if fast else 0) | (0 if prune else _lib.TG_FLAG_NO_PRUNE)
+        check(self.lib.tg_mesher_export_culled(self.h, program.handle, C.byref(g), surrogate.h, flags),
+              "tg_mesher_export_culled")
+        self._cells = tuple(int(c) for c in cells)
+        return self.stats()
+
+    def _bricks(self, fn, name) -> np.ndarray:
+        n = C.c_uint64()
+        check(fn(self.h, None, 0, C.byref(n)), name)
+        out = np.zeros(n.value, dtype=np.uint8)
+        if n.value:
+            check(fn(self.h, out.ctypes.data_as(C.POINTER(C.c_uint8)), n.value, C.byref(n)), name)
+        return out
+
+    def brick_flags(self) -> np.ndarray:
+        """Bricks (8^3 cubes, x fastest) kept by the last culled export."""
+        return self._bricks(self.lib.tg_mesher_brick_flags, "tg_mesher_brick_flags")
+
+    def active_bricks(self) -> np.ndarray:
+        """Bricks holding at least one active cube in the last export."""
+        return self._bricks(self.lib.tg_mesher_active_bricks, "tg_mesher_active_bricks")
+
     def mesh_values(self, values, origin, step, cells) -> dict:
         g = _lib.grid_struct(origin, step, cells)
         v = np.ascontiguousarray(values, dtype=np.float32)

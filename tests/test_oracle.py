@@ -181,8 +181,15 @@ def test_central_gradient_and_far_march_closed_forms(oracle):
     hit, t, steps = oracle.raymarch_far(words, starts, (-dirs).astype(np.float32), 256, 1e-4, 40.0)
     assert hit.all()
     np.testing.assert_allclose(t, 4.5, atol=2e-4)
+    # steps count distance evaluations: 4.5 lands on the surface, the second
+    # evaluation is the hit test
+    assert np.all(steps == 2)
     hit, t, steps = oracle.raymarch_far(words, starts, dirs.astype(np.float32), 256, 1e-4, 40.0)
     assert not hit.any() and np.all(np.isinf(t))
+    # moving away: t = 4.5, 13.5, 31.5, 67.5 > far after the fourth evaluation
+    assert np.all(steps == 4)
+    hit, t, steps = oracle.raymarch_far(words, starts, dirs.astype(np.float32), 3, 1e-4, 40.0)
+    assert not hit.any() and np.all(steps == 3)
     oracle.free(h)
 
 
@@ -235,3 +242,52 @@ def test_reference_harness_agrees(oracle, reference):
         assert np.array_equal(bits(oracle.eval(ho, p)), bits(reference.eval(hr, p)))
         oracle.free(ho)
         reference.free(hr)
+
+
+def _mlp_case(seed, n=64):
+    from oracle import mlp
+    rng = np.random.default_rng(seed)
+    params = rng.normal(0, 0.2, mlp.PARAMS)
+    fourier = rng.normal(0, 2.0, (mlp.FREQ, 3))
+    lo = np.array([-2.0, -1.0, -3.0])
+    xyz = lo + 4.0 * rng.random((n, 3))
+    target = rng.normal(0, 0.6, n)
+    return mlp, params, fourier, lo, xyz, target
+
+
+def test_mlp_oracle_gradient_matches_finite_differences():
+    """The surrogate oracle's backward pass (float64) against central
+    differences of its own loss, on every parameter block."""
+    mlp, params, fourier, lo, xyz, target = _mlp_case(7)
+    loss, g = mlp.loss_and_grad(params, fourier, lo, 4.0, xyz, target, 0.5)
+    assert loss == pytest.approx(mlp.loss_only(params, fourier, lo, 4.0, xyz, target, 0.5), rel=1e-12)
+    rng = np.random.default_rng(1)
+    blocks = [(mlp.W1, mlp.B1), (mlp.B1, mlp.W2), (mlp.W2, mlp.B2), (mlp.B2, mlp.W3), (mlp.W3, mlp.B3),
+              (mlp.B3, mlp.PARAMS)]
+    h = 1e-6
+    for a, b in blocks:
+        for i in rng.choice(np.arange(a, b), size=min(8, b - a), replace=False):
+            p1, p0 = params.copy(), params.copy()
+            p1[i] += h
+            p0[i] -= h
+            fd = (mlp.loss_only(p1, fourier, lo, 4.0, xyz, target, 0.5) -
+                  mlp.loss_only(p0, fourier, lo, 4.0, xyz, target, 0.5)) / (2 * h)
+            assert abs(fd - g[i]) <= 1e-6 + 1e-5 * abs(fd), (a, i, fd, g[i])
+
+
+def test_mlp_oracle_planted_mistake_is_caught():
+    """A transposed W2 in the backward pass would fail the check above."""
+    mlp, params, fourier, lo, xyz, target = _mlp_case(3)
+    _, g = mlp.loss_and_grad(params, fourier, lo, 4.0, xyz, target, 0.5)
+    w2 = params[mlp.W2:mlp.B2].reshape(64, 64)
+    p_t = params.copy()
+    p_t[mlp.W2:mlp.B2] = w2.T.reshape(-1)
+    # gradient of the loss whose W2 is transposed, mapped back: differs from g
+    _, g_t = mlp.loss_and_grad(p_t, fourier, lo, 4.0, xyz, target, 0.5)
+    assert np.max(np.abs(g_t[mlp.W1:mlp.B1] - g[mlp.W1:mlp.B1])) > 1e-4
+    # bf16 rounding emulation: values representable in bf16 are unchanged,
+    # others move by at most half a bf16 ulp
+    v = np.array([1.0, 1.5, -3.25, 1 + 2 ** -9, 1 + 3 * 2 ** -9], dtype=np.float32)
+    r = mlp.bf16(v)
+    assert list(r[:3]) == [1.0, 1.5, -3.25]
+    assert r[3] == 1.0 and r[4] == np.float32(1 + 2 ** -7)
