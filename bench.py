@@ -151,6 +151,7 @@ def surrogate_leg(mesher, prog, wl, origin, step, cells, args) -> dict:
         "identical_mesh": bool(same),
         "missed_bricks": missed,
         "bricks_kept": int(st["bricks_flagged"]),
+        "cull_threshold": float(st["cull_threshold"]),
         "bricks_total": int(st["bricks_total"]),
         "bricks_active": int(active.sum()),
         "exact_evals_culled": int(st["exact_evals"]),

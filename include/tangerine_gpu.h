@@ -115,6 +115,7 @@ typedef struct tg_export_stats
 	uint64_t bricks_total;     /* culled export: 8^3-cube bricks of the grid (0 otherwise) */
 	uint64_t bricks_flagged;   /* culled export: bricks the surrogate kept for exact evaluation */
 	double predict_ms;         /* culled export: surrogate prediction and tile selection (HIP events) */
+	double cull_threshold;     /* culled export: bricks with |prediction at the centre| above this were skipped */
 } tg_export_stats;
 
 typedef struct tg_surrogate_stats
@@ -127,8 +128,8 @@ typedef struct tg_surrogate_stats
 	uint64_t surface_vertices; /* vertices of the coarse exact mesh the near-surface samples start from */
 	uint32_t steps;            /* Adam steps */
 	float final_loss;          /* mean squared error of the last batch */
-	float margin;              /* max |prediction - clamped exact distance| over the calibration samples */
-	float mean_residual;       /* mean of the same */
+	float margin;              /* max |prediction - clamped exact distance| over the calibration samples (reported) */
+	float mean_residual;       /* mean of the same (the cull threshold's slack) */
 } tg_surrogate_stats;
 
 /* ---- errors / device ----------------------------------------------------- */
@@ -239,10 +240,12 @@ int tg_mesher_stats(const tg_mesher* mesher, tg_export_stats* out);
  * A Fourier-feature MLP (3 -> 96 features, sigma-scaled Gaussian frequencies
  * -> 64 SiLU -> 64 SiLU -> 1; BF16 matrix-core operands, FP32 accumulation)
  * fitted to clamp(d, -clamp, clamp) over the cube [lo, lo + size]^3.  Its only
- * use is to choose which 8^3-cube bricks a culled export evaluates exactly:
- * a brick is skipped when |prediction at its centre| > margin + its
- * half-diagonal, where margin is the largest residual seen on held-out
- * calibration samples.  Every sample and vertex of the output still comes
+ * use is to choose which 8^3-cube bricks a culled export evaluates exactly.
+ * A brick of half-diagonal r whose centre c holds surface has |d(c)| <= r,
+ * so it is skipped only when |prediction(c)| exceeds the largest |prediction|
+ * seen at held-out calibration points with |exact d| <= r, plus the mean
+ * calibration residual (or, after tg_surrogate_set_margin, when it exceeds
+ * r + margin).  Every sample and vertex of the output still comes
  * from the exact evaluator; a brick holding surface that the surrogate
  * wrongly skipped would be missing from the mesh, which
  * tg_mesher_active_bricks lets a caller audit against a plain export. */
@@ -263,6 +266,7 @@ int tg_surrogate_gradients(tg_surrogate* surrogate, const float* xyz, const floa
  * `epochs` passes of Adam (batches of 16384) and calibrate the margin. */
 int tg_surrogate_fit(tg_surrogate* surrogate, const tg_program* exact, uint64_t samples, uint64_t calib_samples,
                      float surface_sigma, uint32_t epochs, uint32_t flags, tg_surrogate_stats* stats);
+/* Replace the calibrated threshold by r + margin (margin >= 0). */
 int tg_surrogate_set_margin(tg_surrogate* surrogate, float margin);
 int tg_surrogate_get_stats(const tg_surrogate* surrogate, tg_surrogate_stats* out);
 /* Mean training loss per Adam step of the last fit. */

@@ -39,7 +39,8 @@ class TgProgramInfo(C.Structure):
 class TgExportStats(C.Structure):
     _fields_ = [("eval_ms", C.c_double), ("mesh_ms", C.c_double), ("total_ms", C.c_double),
                 ("samples", _u64), ("tiles_evaluated", _u64), ("exact_evals", _u64), ("flops", C.c_double),
-                ("bricks_total", _u64), ("bricks_flagged", _u64), ("predict_ms", C.c_double)]
+                ("bricks_total", _u64), ("bricks_flagged", _u64), ("predict_ms", C.c_double),
+                ("cull_threshold", C.c_double)]
 
 
 class TgSurrogateStats(C.Structure):

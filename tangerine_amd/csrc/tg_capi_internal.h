@@ -44,6 +44,9 @@ struct CellPrograms
 	uint64_t ncells = 0;
 	uint64_t entries = 0;     // pool entries in use
 	uint32_t max_stack = 0;   // over all cells
+	uint32_t max_binary = 0;  // most binary entries of a packed cell program
+	uint32_t max_len = 0;     // most entries of a packed cell program
+	uint64_t whole_cells = 0; // cells left with the whole program (pool overflow)
 	bool valid = false;       // false: the program is not prunable here
 	void release()
 	{
@@ -65,5 +68,6 @@ namespace tg {
 // Device vertices of the mesher's last export (3 floats each).
 const float* mesher_vertices(const tg_mesher* m);
 mlp::Net surrogate_net(const tg_surrogate* s);
-float surrogate_margin(const tg_surrogate* s);
+// Cull threshold for bricks of half-diagonal r (< 0: not calibrated).
+float surrogate_threshold(const tg_surrogate* s, float r);
 }

@@ -40,7 +40,7 @@ struct tg_mesher
 	bool have_grid = false;
 	tg_export_stats stats{};
 	int points_per_lane = 8;      // preferred points per lane (tuned)
-	size_t lds_budget = 80 * 1024;  // LDS per workgroup the block kernel may use
+	size_t lds_budget = 40 * 1024;  // LDS per workgroup the block kernel aims for (4 workgroups per CU)
 	int last_points_per_lane = 0;
 };
 
@@ -247,10 +247,18 @@ int eval_pruned(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fl
 	ba.work = work;
 	ba.whole = p->entries.as<const Entry>();
 	ba.n_whole = uint32_t(p->decoded.entries.size());
+	ba.n_whole_binary = p->decoded.binary_entries;
+	ba.max_binary = m->blocks.max_binary;
+	ba.max_len = m->blocks.max_len;
 	ba.waves = 4;
 	int pts = m->points_per_lane;
-	while (pts > 2 && block_lds_bytes(ba.slots, pts, ba.waves) > m->lds_budget) pts >>= 1;
-	if (block_lds_bytes(ba.slots, pts, ba.waves) > kMaxLds) return 0;
+	while (pts > 2 && block_lds_bytes(ba.slots, pts, ba.waves, ba.max_binary, ba.max_len) > m->lds_budget) pts >>= 1;
+	if (block_lds_bytes(ba.slots, pts, ba.waves, ba.max_binary, ba.max_len) > kMaxLds)
+	{
+		// Block programs too long for per-tile lists in LDS: interpret them whole.
+		ba.prune_tiles = 0;
+		ba.max_binary = ba.max_len = 0;
+	}
 	m->last_points_per_lane = pts;
 	hipError_t e = (flags & TG_FLAG_FAST) ? fast::launch_eval_blocks(ba, pts, m->stream)
 	                                      : exact::launch_eval_blocks(ba, pts, m->stream);
@@ -302,7 +310,7 @@ namespace {
 // Surrogate stage of a culled export: brick flags, per-block tile masks and
 // the list of blocks (or, for the unpruned evaluator, tiles) to evaluate.
 int select_tiles(tg_mesher* m, const tg_grid* g, const tg_surrogate* s, bool tiles, uint32_t* listed,
-	uint64_t* flagged)
+	uint64_t* flagged, float* threshold)
 {
 	MeshArgs& ma = m->args;
 	const uint32_t bx = (ma.tx + kBlockTiles - 1) / kBlockTiles, by = (ma.ty + kBlockTiles - 1) / kBlockTiles,
@@ -327,7 +335,8 @@ int select_tiles(tg_mesher* m, const tg_grid* g, const tg_surrogate* s, bool til
 	pa.dx = g->dx; pa.dy = g->dy; pa.dz = g->dz;
 	pa.cbx = ma.cbx; pa.cby = ma.cby; pa.cbz = ma.cbz;
 	pa.radius = 4.0f * std::sqrt(g->dx * g->dx + g->dy * g->dy + g->dz * g->dz);
-	pa.threshold = surrogate_margin(s) + pa.radius;
+	pa.threshold = surrogate_threshold(s, pa.radius);
+	*threshold = pa.threshold;
 	pa.flags = m->brick_flags.as<uint8_t>();
 	if (!(pa.threshold < pa.net.clamp))
 	{
@@ -365,7 +374,8 @@ int run_export(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fla
 	{
 		if (!p->decoded.lipschitz)
 			return fail(TG_ERR_PROGRAM, "culled export needs a 1-Lipschitz program (no ellipsoid, no stretching scale)");
-		if (!(surrogate_margin(sur) >= 0.0f)) return fail(TG_ERR_INVALID, "culled export: surrogate is not calibrated");
+		if (!(surrogate_threshold(sur, 0.0f) >= 0.0f))
+			return fail(TG_ERR_INVALID, "culled export: surrogate is not calibrated");
 	}
 	m->have_grid = false;
 	m->nbricks = 0;
@@ -390,9 +400,10 @@ int run_export(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fla
 	TG_CHECK_HIP(hipEventRecord(m->ev[0], m->stream), "event");
 	uint32_t listed = 0;
 	uint64_t flagged = 0;
+	float threshold = 0.0f;
 	if (sur)
 	{
-		rc = select_tiles(m, g, sur, !try_blocks, &listed, &flagged);
+		rc = select_tiles(m, g, sur, !try_blocks, &listed, &flagged, &threshold);
 		if (rc) return rc;
 	}
 	TG_CHECK_HIP(hipEventRecord(m->ev[3], m->stream), "event");
@@ -408,7 +419,7 @@ int run_export(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fla
 		if (!pruned && sur)
 		{
 			// Block evaluator unavailable (LDS): list tiles for the plain one.
-			rc = select_tiles(m, g, sur, true, &listed, &flagged);
+			rc = select_tiles(m, g, sur, true, &listed, &flagged, &threshold);
 			if (rc) return rc;
 		}
 	}
@@ -449,6 +460,7 @@ int run_export(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fla
 		s.bricks_total = m->nbricks;
 		s.bricks_flagged = flagged;
 		s.predict_ms = t_pred;
+		s.cull_threshold = threshold;
 	}
 	m->have_grid = sur == nullptr;
 	return TG_OK;

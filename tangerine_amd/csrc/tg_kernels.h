@@ -70,8 +70,9 @@ struct BlockDesc
 // (cell (i,j,k) centred at origin + (i+.5, j+.5, k+.5) * cell, valid within
 // `radius` of its centre).  Pruned programs are packed into `pool`; a cell
 // whose program does not fit (or whose pruning walk overflows) is marked to
-// run the whole program.  `counters[0]` accumulates the pool entries asked
-// for, `counters[1]` the stack high-water mark over cells.
+// run the whole program.  counters: [0] pool entries asked for, [1] stack
+// high-water mark over cells, [2] / [3] most binary entries / entries of a
+// packed program, [4] cells left with the whole program.
 struct PruneArgs
 {
 	const Instr* prog;
@@ -112,6 +113,9 @@ struct BlockEvalArgs
 	unsigned long long* work;    // [0] samples interpreted, [1] flops units executed
 	const Entry* whole;          // whole program, for blocks whose pruning overflowed
 	uint32_t n_whole;
+	uint32_t n_whole_binary;
+	uint32_t max_binary;         // LDS tables: binary entries / entries of any block program
+	uint32_t max_len;
 	int waves;                   // waves per workgroup
 };
 
@@ -186,18 +190,26 @@ hipError_t launch_trace(const TraceArgs& a, hipStream_t s);
 hipError_t launch_gradient(const GradArgs& a, hipStream_t s);
 }
 
-// LDS layout of the block kernel (bytes), shared by launcher and kernel.
-constexpr uint32_t kMaxBlockBinary = 1024;     // decision masks held for a block program
-constexpr uint32_t kTileListCap = 256;         // entries of a pruned tile program
-constexpr uint32_t kDfsCap = 128;              // DFS frames per wave
-__host__ __device__ inline size_t block_lds_bytes(uint32_t slots, int p, int waves)
+// LDS layout of the block kernel (bytes), shared by launcher and kernel:
+// decision masks (2 x max_binary) and liveness (max_len) of the block program,
+// then per wave a tile program list (max_len entries) and the spill region.
+__host__ __device__ inline size_t block_table_bytes(uint32_t max_binary, uint32_t max_len)
 {
-	return size_t(kMaxBlockBinary) * 16 + size_t(waves) * (size_t(kTileListCap) * 16 + size_t(kDfsCap) * 16 +
-	                                                        spill_bytes_per_wave(slots, p));
+	return size_t(max_binary) * 16 + ((size_t(max_len) * 8 + 15) & ~size_t(15));
 }
-__host__ __device__ inline size_t prune_lds_bytes(uint32_t n_binary, uint32_t slots)
+__host__ __device__ inline size_t block_wave_bytes(uint32_t slots, int p, uint32_t max_len)
 {
-	return size_t(n_binary) * 16 + size_t(kDfsCap) * 16 + spill_bytes_per_wave(slots, 1);
+	return size_t(max_len) * 16 + spill_bytes_per_wave(slots, p);
+}
+__host__ __device__ inline size_t block_lds_bytes(uint32_t slots, int p, int waves, uint32_t max_binary,
+	uint32_t max_len)
+{
+	return block_table_bytes(max_binary, max_len) + size_t(waves) * block_wave_bytes(slots, p, max_len);
+}
+// Cell pruning: decision masks, liveness, per-lane position stacks, spill.
+__host__ __device__ inline size_t prune_lds_bytes(uint32_t n_binary, uint32_t n_entries, uint32_t slots)
+{
+	return size_t(n_binary) * 16 + size_t(n_entries) * 8 + size_t(kMaxStack) * 64 * 4 + spill_bytes_per_wave(slots, 1);
 }
 
 }  // namespace tg

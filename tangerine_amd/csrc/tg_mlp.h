@@ -43,7 +43,7 @@ struct PredictArgs
 	// brick flagging (xyz == null): cube bricks of 8^3 cells of a grid
 	float gx, gy, gz, dx, dy, dz;
 	uint32_t cbx, cby, cbz;   // cube bricks per axis
-	float threshold;          // flag if |prediction| <= threshold (= margin + brick half-diagonal)
+	float threshold;          // keep a brick unless |prediction at its centre| > threshold
 	float radius;             // brick half-diagonal: bricks reaching outside the domain are flagged
 	uint8_t* flags;
 };

@@ -259,7 +259,8 @@ __global__ void __launch_bounds__(256) k_predict(PredictArgs p)
 					const float uz = fabsf((pz - p.net.lo[2]) * p.net.inv_half - 1.0f);
 					const float reach = p.radius * p.net.inv_half;
 					const bool outside = fmaxf(ux, fmaxf(uy, uz)) + reach > 1.0f;
-					p.flags[k] = (outside || !(fabsf(y) <= p.threshold)) ? 1 : 0;
+					// keep unless the prediction certifies distance (NaN keeps)
+					p.flags[k] = (outside || !(fabsf(y) > p.threshold)) ? 1 : 0;
 				}
 			}
 		}

@@ -17,7 +17,7 @@ int prune_cells(hipStream_t stream, const tg_program* p, const float origin[3], 
 	out.valid = false;
 	if (!d.lipschitz || d.entries.empty() || d.entries.size() >= 65535) return TG_OK;
 	const uint32_t whole_slots = std::max<uint32_t>(1, d.max_stack - 1);
-	if (prune_lds_bytes(d.binary_entries, whole_slots) > 64 * 1024) return TG_OK;
+	if (prune_lds_bytes(d.binary_entries, uint32_t(d.entries.size()), whole_slots) > 160 * 1024) return TG_OK;
 	const uint64_t ncells = uint64_t(counts[0]) * counts[1] * counts[2];
 	if (ncells == 0 || ncells >= (1ull << 31)) return TG_OK;
 	int rc = out.desc.ensure(ncells * sizeof(BlockDesc));
@@ -50,11 +50,14 @@ int prune_cells(hipStream_t stream, const tg_program* p, const float origin[3], 
 		pa.counters = out.counters.as<unsigned long long>();
 		e = fast ? fast::launch_prune_cells(pa, stream) : exact::launch_prune_cells(pa, stream);
 		if (e != hipSuccess) return hip_fail(e, "cell pruning launch");
-		unsigned long long c[2] = { 0, 0 };
-		e = hipMemcpyAsync(c, out.counters.ptr, 16, hipMemcpyDeviceToHost, stream);
+		unsigned long long c[5] = { 0, 0, 0, 0, 0 };
+		e = hipMemcpyAsync(c, out.counters.ptr, sizeof(c), hipMemcpyDeviceToHost, stream);
 		if (e == hipSuccess) e = hipStreamSynchronize(stream);
 		if (e != hipSuccess) return hip_fail(e, "cell pruning");
 		out.max_stack = uint32_t(c[1]);
+		out.max_binary = uint32_t(c[2]);
+		out.max_len = uint32_t(c[3]);
+		out.whole_cells = c[4];
 		out.entries = c[0];
 		if (c[0] <= capacity || c[0] * sizeof(Entry) > (size_t(8) << 30)) break;
 		capacity = c[0];

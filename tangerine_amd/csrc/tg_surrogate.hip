@@ -4,8 +4,14 @@
 //         near the surface: vertices of a coarse exact export plus Gaussian
 //         offsets), labelled by the exact evaluator, and fitted with Adam;
 //         the training steps of one epoch are captured in a hipGraph and
-//         replayed.  The culling margin is the largest |prediction - clamped
-//         exact distance| over separately drawn calibration samples.
+//         replayed.  Separately drawn calibration samples (prediction and
+//         exact distance) are kept on the host.
+// threshold(r): a brick of half-diagonal r whose centre c holds surface has
+//         |d(c)| <= r (the field is 1-Lipschitz), so the brick may be skipped
+//         only if |prediction(c)| exceeds every prediction seen at calibration
+//         points with |d| <= r.  The threshold is that maximum (at least r)
+//         plus the mean calibration residual as slack; a margin set by the
+//         caller replaces the calibration: threshold = r + margin.
 // The culled export (tg_export.hip) evaluates exactly every tile that a
 // flagged brick touches, so the surrogate only decides which work is skipped.
 
@@ -42,7 +48,10 @@ struct tg_surrogate
 	float sigma = 2.0f;
 	float clamp = 0.5f;
 	uint64_t seed = 1;
-	float margin = -1.0f;       // < 0: not calibrated
+	float margin = -1.0f;       // caller's fixed margin (< 0: use the calibration)
+	bool calibrated = false;
+	std::vector<float> calib_pred, calib_dist;  // held-out predictions and exact distances
+	float mean_residual = 0.0f;
 	DevBuf params, fourier, adam_m, adam_v, xyz, target, order, partial, loss_partial, counters, loss_log, tmp;
 	tg_surrogate_stats stats{};
 };
@@ -60,7 +69,16 @@ mlp::Net surrogate_net(const tg_surrogate* s)
 	return n;
 }
 
-float surrogate_margin(const tg_surrogate* s) { return s->margin; }
+float surrogate_threshold(const tg_surrogate* s, float r)
+{
+	if (s->margin >= 0.0f) return r + s->margin;
+	if (!s->calibrated) return -1.0f;
+	float t = r;
+	const size_t n = s->calib_pred.size();
+	for (size_t i = 0; i < n; ++i)
+		if (std::fabs(s->calib_dist[i]) <= r) t = std::max(t, std::fabs(s->calib_pred[i]));
+	return t + s->mean_residual;
+}
 
 }  // namespace tg
 
@@ -417,7 +435,11 @@ extern "C" int tg_surrogate_fit(tg_surrogate* s, const tg_program* exact, uint64
 		max_res = std::max(max_res, r);
 		sum_res += r;
 	}
-	s->margin = float(max_res);
+	s->calib_pred = std::move(pred);
+	s->calib_dist = std::move(truth);
+	s->mean_residual = float(sum_res / double(calib_samples));
+	s->calibrated = true;
+	s->margin = -1.0f;
 	auto t2 = std::chrono::steady_clock::now();
 	tg_surrogate_stats st{};
 	st.sample_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
@@ -427,8 +449,8 @@ extern "C" int tg_surrogate_fit(tg_surrogate* s, const tg_program* exact, uint64
 	st.steps = uint32_t(total_steps);
 	st.final_loss = final_loss;
 	st.calib_samples = calib_samples;
-	st.margin = s->margin;
-	st.mean_residual = float(sum_res / double(calib_samples));
+	st.margin = float(max_res);
+	st.mean_residual = s->mean_residual;
 	st.surface_vertices = nverts;
 	s->stats = st;
 	if (stats) *stats = st;
