@@ -124,37 +124,55 @@ def surrogate_leg(mesher, prog, wl, origin, step, cells, args) -> dict:
     log(f"surrogate fit: {fit_wall:.2f} s, margin {fit['margin']:.4f}, mean residual {fit['mean_residual']:.4f}, "
         f"loss {loss[0]:.4g} -> {loss[-1]:.4g}")
 
-    def culled_once():
-        return mesher.export_culled(prog, sur, origin, step, cells, fast=args.fast, prune=not args.no_prune)
+    def culled_once(surrogate, certify):
+        return mesher.export_culled(prog, surrogate, origin, step, cells, fast=args.fast, prune=not args.no_prune,
+                                    certify=certify)
 
-    for i in range(args.warmup):
-        st = culled_once()
-        log(f"culled warmup {i}: predict {st['predict_ms']:.1f} eval {st['eval_ms']:.1f} mesh {st['mesh_ms']:.1f} ms")
-    totals, evals_ms = [], []
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        st = culled_once()
-        totals.append(st["total_ms"])
-        evals_ms.append(st["eval_ms"])
-        log(f"culled step {i}: predict {st['predict_ms']:.1f} eval {st['eval_ms']:.1f} mesh {st['mesh_ms']:.1f} "
-            f"total {st['total_ms']:.1f} ms, kept {st['bricks_flagged']}/{st['bricks_total']} bricks")
-    _lib.check(_lib.load().tg_device_synchronize(), "sync")
-    wall = time.perf_counter() - t0
-    kept = mesher.brick_flags().astype(bool)
-    missed = int(np.count_nonzero(active & ~kept))
-    same = digest(mesher) == plain_digest
+    def timed(surrogate, certify, label):
+        for i in range(args.warmup):
+            culled_once(surrogate, certify)
+        totals, evals_ms = [], []
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            st = culled_once(surrogate, certify)
+            totals.append(st["total_ms"])
+            evals_ms.append(st["eval_ms"])
+            log(f"{label} step {i}: select+prune {st['prune_ms']:.1f} eval {st['eval_ms']:.1f} mesh "
+                f"{st['mesh_ms']:.1f} total {st['total_ms']:.1f} ms, kept {st['bricks_flagged']}/{st['bricks_total']} "
+                f"bricks ({st['bricks_certified']} by the certificate)")
+        _lib.check(_lib.load().tg_device_synchronize(), "sync")
+        wall = time.perf_counter() - t0
+        kept = mesher.brick_flags().astype(bool)
+        res = {
+            "export_ms": 1000.0 * wall / args.steps,
+            "export_ms_device": float(np.mean(totals)),
+            "eval_ms": float(np.mean(evals_ms)),
+            "identical_mesh": bool(digest(mesher) == plain_digest),
+            "missed_bricks": int(np.count_nonzero(active & ~kept)),
+            "bricks_kept": int(st["bricks_flagged"]),
+            "bricks_certified": int(st["bricks_certified"]),
+            "exact_evals": int(st["exact_evals"]),
+        }
+        return res, st
+
+    guess, _ = timed(sur, False, "surrogate alone")
+    cert_only, _ = timed(None, True, "certificate alone")
+    res, st = timed(sur, True, "surrogate + certificate")
     sur.close()
     return {
-        "export_ms_culled": 1000.0 * wall / args.steps,
-        "export_ms_culled_device": float(np.mean(totals)),
-        "eval_ms_culled": float(np.mean(evals_ms)),
-        "identical_mesh": bool(same),
-        "missed_bricks": missed,
-        "bricks_kept": int(st["bricks_flagged"]),
-        "cull_threshold": float(st["cull_threshold"]),
+        "export_ms_culled": res["export_ms"],
+        "export_ms_culled_device": res["export_ms_device"],
+        "eval_ms_culled": res["eval_ms"],
+        "identical_mesh": res["identical_mesh"],
+        "missed_bricks": res["missed_bricks"],
+        "bricks_kept": res["bricks_kept"],
+        "bricks_certified": res["bricks_certified"],
         "bricks_total": int(st["bricks_total"]),
         "bricks_active": int(active.sum()),
-        "exact_evals_culled": int(st["exact_evals"]),
+        "cull_threshold": float(st["cull_threshold"]),
+        "exact_evals_culled": res["exact_evals"],
+        "surrogate_alone": guess,
+        "certificate_alone": cert_only,
         "fit": {"wall_ms": 1000.0 * fit_wall, "train_ms": fit["train_ms"], "sample_ms": fit["sample_ms"],
                 "samples": fit["samples"], "calib_samples": fit["calib_samples"], "steps": fit["steps"],
                 "margin": fit["margin"], "mean_residual": fit["mean_residual"],
@@ -173,6 +191,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-prune", action="store_true", help="interpret the whole program in every tile")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate-culled export leg")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default block-kernel launch")
     ap.add_argument("--sur-samples", type=int, default=2 * 1024 * 1024)
     ap.add_argument("--sur-epochs", type=int, default=16)
     ap.add_argument("--sur-clamp", type=float, default=0.5)
@@ -201,6 +220,14 @@ def main(argv=None) -> int:
 
     def step_once():
         return mesher.export(prog, origin, step, cells, fast=args.fast, prune=prune)
+
+    tuning = None
+    if prune and not args.no_autotune:
+        t0 = time.perf_counter()
+        tuning = mesher.autotune(prog, origin, step, cells, fast=args.fast)
+        tuning["wall_ms"] = 1000.0 * (time.perf_counter() - t0)
+        log(f"autotune: {tuning['points_per_lane']} points/lane, {tuning['waves']} waves, "
+            f"{tuning['eval_ms']:.2f} ms ({tuning['wall_ms']:.0f} ms to tune)")
 
     for i in range(args.warmup):
         st = step_once()
@@ -252,6 +279,7 @@ def main(argv=None) -> int:
         "mesh_ms": float(np.mean(mesh_ms)),
         "samples_per_s": samples / (ms_per_step * 1e-3),
         "mesh": {"vertices": nverts, "triangles": ntris},
+        "autotune": tuning,
         "surrogate_culled": culled,
         "roofline": {
             "bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",

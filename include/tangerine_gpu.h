@@ -53,6 +53,7 @@ extern "C" {
 /* Evaluation flags. */
 #define TG_FLAG_FAST 1u      /* contracted FP32 numerics; default is bit parity with SDFInterpreter::Eval */
 #define TG_FLAG_NO_PRUNE 2u  /* interpret the whole program in every tile (no per-brick pruning) */
+#define TG_FLAG_CERTIFY 4u   /* culled export: exact Lipschitz certificate for every skipped brick */
 
 typedef struct tg_node tg_node;
 typedef struct tg_program tg_program;
@@ -114,8 +115,9 @@ typedef struct tg_export_stats
 	double flops;              /* algorithmic FP32 work executed (cost table) */
 	uint64_t bricks_total;     /* culled export: 8^3-cube bricks of the grid (0 otherwise) */
 	uint64_t bricks_flagged;   /* culled export: bricks the surrogate kept for exact evaluation */
-	double predict_ms;         /* culled export: surrogate prediction and tile selection (HIP events) */
+	double prune_ms;           /* block pruning, and for a culled export brick selection (HIP events; inside eval_ms) */
 	double cull_threshold;     /* culled export: bricks with |prediction at the centre| above this were skipped */
+	uint64_t bricks_certified; /* culled export: bricks the exact certificate kept that the surrogate skipped */
 } tg_export_stats;
 
 typedef struct tg_surrogate_stats
@@ -236,6 +238,26 @@ int tg_mesher_copy_mesh(const tg_mesher* mesher, float* vertices, uint32_t* tria
 int tg_mesher_copy_grid(const tg_mesher* mesher, float* values);
 int tg_mesher_stats(const tg_mesher* mesher, tg_export_stats* out);
 
+/* ---- block-evaluator tuning (no reference counterpart) ----------------------
+ * The pruned block kernel has two launch parameters: points per lane (2, 4
+ * or 8 samples interpreted together by one lane: more work per fetched
+ * instruction, more registers and LDS) and waves per workgroup (1, 2, 4).
+ * tg_mesher_autotune prunes `program` for `grid`, times every combination on
+ * the device and keeps the fastest for later exports on this mesher. */
+typedef struct tg_tune_result
+{
+	uint32_t points_per_lane;     /* chosen */
+	uint32_t waves;
+	double eval_ms;               /* block kernel time of the chosen configuration */
+	uint32_t candidates;
+	uint32_t candidate_points[9]; /* configuration that actually ran (LDS may force a smaller one) */
+	uint32_t candidate_waves[9];
+	double candidate_ms[9];
+} tg_tune_result;
+int tg_mesher_autotune(tg_mesher* mesher, const tg_program* program, const tg_grid* grid, uint32_t flags,
+                       tg_tune_result* out);
+int tg_mesher_set_tuning(tg_mesher* mesher, uint32_t points_per_lane, uint32_t waves);
+
 /* ---- learned brick culling (no reference counterpart) ----------------------
  * A Fourier-feature MLP (3 -> 96 features, sigma-scaled Gaussian frequencies
  * -> 64 SiLU -> 64 SiLU -> 1; BF16 matrix-core operands, FP32 accumulation)
@@ -271,9 +293,13 @@ int tg_surrogate_set_margin(tg_surrogate* surrogate, float margin);
 int tg_surrogate_get_stats(const tg_surrogate* surrogate, tg_surrogate_stats* out);
 /* Mean training loss per Adam step of the last fit. */
 int tg_surrogate_loss_log(const tg_surrogate* surrogate, float* out, uint64_t capacity, uint64_t* count);
-/* Export that evaluates exactly only the tiles of bricks the surrogate keeps.
- * Needs a 1-Lipschitz program and a calibrated surrogate.  The samples of
- * skipped tiles are not computed, so tg_mesher_copy_grid refuses afterwards. */
+/* Export that evaluates exactly only the tiles of kept bricks.  Needs a
+ * 1-Lipschitz program.  With a calibrated surrogate, bricks are kept by its
+ * prediction; with TG_FLAG_CERTIFY (or surrogate = NULL) the block's pruned
+ * program is also evaluated exactly at every skipped brick's centre c, and a
+ * brick with |d(c)| <= its half-diagonal (+ rounding slack) is kept, so no
+ * brick holding a sign change can be skipped.  The samples of skipped tiles
+ * are not computed, so tg_mesher_copy_grid refuses afterwards. */
 int tg_mesher_export_culled(tg_mesher* mesher, const tg_program* program, const tg_grid* grid,
                             const tg_surrogate* surrogate, uint32_t flags);
 /* One byte per 8^3-cube brick (x fastest): the kept bricks of the last culled

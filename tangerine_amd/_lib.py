@@ -16,6 +16,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libtangerine_gpu.so"
 TG_OK = 0
 TG_FLAG_FAST = 1
 TG_FLAG_NO_PRUNE = 2
+TG_FLAG_CERTIFY = 4
 
 _f = C.c_float
 _u32 = C.c_uint32
@@ -39,14 +40,19 @@ class TgProgramInfo(C.Structure):
 class TgExportStats(C.Structure):
     _fields_ = [("eval_ms", C.c_double), ("mesh_ms", C.c_double), ("total_ms", C.c_double),
                 ("samples", _u64), ("tiles_evaluated", _u64), ("exact_evals", _u64), ("flops", C.c_double),
-                ("bricks_total", _u64), ("bricks_flagged", _u64), ("predict_ms", C.c_double),
-                ("cull_threshold", C.c_double)]
+                ("bricks_total", _u64), ("bricks_flagged", _u64), ("prune_ms", C.c_double),
+                ("cull_threshold", C.c_double), ("bricks_certified", _u64)]
 
 
 class TgSurrogateStats(C.Structure):
     _fields_ = [("sample_ms", C.c_double), ("train_ms", C.c_double), ("total_ms", C.c_double),
                 ("samples", _u64), ("calib_samples", _u64), ("surface_vertices", _u64), ("steps", _u32),
                 ("final_loss", _f), ("margin", _f), ("mean_residual", _f)]
+
+
+class TgTuneResult(C.Structure):
+    _fields_ = [("points_per_lane", _u32), ("waves", _u32), ("eval_ms", C.c_double), ("candidates", _u32),
+                ("candidate_points", _u32 * 9), ("candidate_waves", _u32 * 9), ("candidate_ms", C.c_double * 9)]
 
 
 class TgCamera(C.Structure):
@@ -116,6 +122,8 @@ SIGNATURES = {
     "tg_mesher_copy_mesh": (C.c_int, [_p, _fp, _u32p]),
     "tg_mesher_copy_grid": (C.c_int, [_p, _fp]),
     "tg_mesher_stats": (C.c_int, [_p, C.POINTER(TgExportStats)]),
+    "tg_mesher_autotune": (C.c_int, [_p, _p, C.POINTER(TgGrid), _u32, C.POINTER(TgTuneResult)]),
+    "tg_mesher_set_tuning": (C.c_int, [_p, _u32, _u32]),
     "tg_mesher_export_culled": (C.c_int, [_p, _p, C.POINTER(TgGrid), _p, _u32]),
     "tg_mesher_brick_flags": (C.c_int, [_p, C.POINTER(C.c_uint8), _u64, _u64p]),
     "tg_mesher_active_bricks": (C.c_int, [_p, C.POINTER(C.c_uint8), _u64, _u64p]),

@@ -39,9 +39,10 @@ struct tg_mesher
 	uint64_t nverts = 0, ntris = 0;
 	bool have_grid = false;
 	tg_export_stats stats{};
-	int points_per_lane = 8;      // preferred points per lane (tuned)
-	size_t lds_budget = 40 * 1024;  // LDS per workgroup the block kernel aims for (4 workgroups per CU)
+	int points_per_lane = 4;      // block kernel: points per lane (2, 4, 8; tg_mesher_autotune)
+	int waves = 4;                // block kernel: waves per workgroup (1, 2, 4)
 	int last_points_per_lane = 0;
+	int last_waves = 0;
 };
 
 namespace {
@@ -202,26 +203,30 @@ __global__ void __launch_bounds__(256) k_active_bricks(const uint64_t* active, u
 
 unsigned grid_for(uint64_t n) { return unsigned((n + 255) / 256); }
 
-// Two-level pruned evaluation of every tile (k_prune_cells over 32^3-sample
-// blocks, then k_eval_blocks).  Returns 1 if it ran, 0 if the program is not
-// prunable (the caller then interprets the whole program per tile), <0 on
-// error.
-int eval_pruned(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t flags, const uint64_t* tile_mask,
+// Level-1 pruning: the program pruned for every 32^3-sample block (cell grid
+// with origin g - d/2, cell 32 d; the samples lie within 15.5 |d| of a block
+// centre).  m->blocks.valid is false when the program is not prunable.
+int prune_blocks(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t flags)
+{
+	const MeshArgs& ma = m->args;
+	const uint32_t counts[3] = { (ma.tx + kBlockTiles - 1) / kBlockTiles, (ma.ty + kBlockTiles - 1) / kBlockTiles,
+	                             (ma.tz + kBlockTiles - 1) / kBlockTiles };
+	const float origin[3] = { g->x - 0.5f * g->dx, g->y - 0.5f * g->dy, g->z - 0.5f * g->dz };
+	const float cell[3] = { float(kBlockSamples) * g->dx, float(kBlockSamples) * g->dy, float(kBlockSamples) * g->dz };
+	const float half = 0.5f * float(kBlockSamples - 1);
+	const float radius = half * std::sqrt(g->dx * g->dx + g->dy * g->dy + g->dz * g->dz);
+	return prune_cells(m->stream, p, origin, cell, radius, counts, (flags & TG_FLAG_FAST) != 0, m->blocks);
+}
+
+// Pruned evaluation (k_eval_blocks) of the listed blocks' masked tiles, or of
+// every tile.  Needs m->blocks.valid.
+int eval_blocks(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t flags, const uint64_t* tile_mask,
 	const uint32_t* block_list, uint32_t n_listed, uint64_t* evals, double* flops)
 {
 	MeshArgs& ma = m->args;
 	const uint32_t counts[3] = { (ma.tx + kBlockTiles - 1) / kBlockTiles, (ma.ty + kBlockTiles - 1) / kBlockTiles,
 	                             (ma.tz + kBlockTiles - 1) / kBlockTiles };
-	// Block (32 samples per axis) centred at g + (32 b + 15.5) d: a cell grid with
-	// origin g - d/2 and cell 32 d; the samples lie within 15.5 |d| of the centre.
-	const float origin[3] = { g->x - 0.5f * g->dx, g->y - 0.5f * g->dy, g->z - 0.5f * g->dz };
-	const float cell[3] = { float(kBlockSamples) * g->dx, float(kBlockSamples) * g->dy, float(kBlockSamples) * g->dz };
-	const float half = 0.5f * float(kBlockSamples - 1);
-	const float radius = half * std::sqrt(g->dx * g->dx + g->dy * g->dy + g->dz * g->dz);
-	int rc = prune_cells(m->stream, p, origin, cell, radius, counts, (flags & TG_FLAG_FAST) != 0, m->blocks);
-	if (rc) return rc;
-	if (!m->blocks.valid) return 0;
-	rc = m->counters.ensure(64);
+	int rc = m->counters.ensure(64);
 	if (rc) return rc;
 	unsigned long long* work = m->counters.as<unsigned long long>();
 	TG_CHECK_HIP(hipMemsetAsync(m->counters.ptr, 0, 64, m->stream), "counter reset");
@@ -250,9 +255,11 @@ int eval_pruned(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fl
 	ba.n_whole_binary = p->decoded.binary_entries;
 	ba.max_binary = m->blocks.max_binary;
 	ba.max_len = m->blocks.max_len;
-	ba.waves = 4;
+	ba.waves = m->waves;
 	int pts = m->points_per_lane;
-	while (pts > 2 && block_lds_bytes(ba.slots, pts, ba.waves, ba.max_binary, ba.max_len) > m->lds_budget) pts >>= 1;
+	// Fit the LDS: fewer points per lane first, then fewer waves.
+	while (pts > 2 && block_lds_bytes(ba.slots, pts, ba.waves, ba.max_binary, ba.max_len) > kMaxLds) pts >>= 1;
+	while (ba.waves > 1 && block_lds_bytes(ba.slots, pts, ba.waves, ba.max_binary, ba.max_len) > kMaxLds) ba.waves >>= 1;
 	if (block_lds_bytes(ba.slots, pts, ba.waves, ba.max_binary, ba.max_len) > kMaxLds)
 	{
 		// Block programs too long for per-tile lists in LDS: interpret them whole.
@@ -260,6 +267,7 @@ int eval_pruned(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fl
 		ba.max_binary = ba.max_len = 0;
 	}
 	m->last_points_per_lane = pts;
+	m->last_waves = ba.waves;
 	hipError_t e = (flags & TG_FLAG_FAST) ? fast::launch_eval_blocks(ba, pts, m->stream)
 	                                      : exact::launch_eval_blocks(ba, pts, m->stream);
 	if (e != hipSuccess) return hip_fail(e, "block evaluation launch");
@@ -268,7 +276,7 @@ int eval_pruned(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fl
 	TG_CHECK_HIP(hipStreamSynchronize(m->stream), "block evaluation");
 	*evals = wk[0];
 	*flops = double(wk[1]);
-	return 1;
+	return TG_OK;
 }
 
 }  // namespace
@@ -307,10 +315,21 @@ extern "C" void tg_mesher_destroy(tg_mesher* m)
 
 namespace {
 
-// Surrogate stage of a culled export: brick flags, per-block tile masks and
-// the list of blocks (or, for the unpruned evaluator, tiles) to evaluate.
-int select_tiles(tg_mesher* m, const tg_grid* g, const tg_surrogate* s, bool tiles, uint32_t* listed,
-	uint64_t* flagged, float* threshold)
+// Brick selection of a culled export: brick flags from the surrogate (or none)
+// plus, with the certificate, every brick whose exact centre value allows a
+// sign change; then per-block tile masks and the list of blocks (or, for the
+// unpruned evaluator, tiles) to evaluate.
+struct Selection
+{
+	uint32_t listed = 0;
+	uint64_t flagged = 0;      // kept bricks
+	uint64_t certified = 0;    // bricks the certificate added
+	float threshold = 0.0f;
+	double cert_flops = 0.0;
+};
+
+int select_tiles(tg_mesher* m, const tg_program* p, const tg_grid* g, const tg_surrogate* s, bool certify, bool tiles,
+	uint32_t flags, Selection& sel)
 {
 	MeshArgs& ma = m->args;
 	const uint32_t bx = (ma.tx + kBlockTiles - 1) / kBlockTiles, by = (ma.ty + kBlockTiles - 1) / kBlockTiles,
@@ -326,25 +345,53 @@ int select_tiles(tg_mesher* m, const tg_grid* g, const tg_surrogate* s, bool til
 	unsigned long long* ctr = m->counters.as<unsigned long long>();
 	TG_CHECK_HIP(hipMemsetAsync(m->counters.ptr, 0, 64, m->stream), "counter reset");
 	TG_CHECK_HIP(hipMemsetAsync(m->tile_mask.ptr, 0, nblocks * 8, m->stream), "mask reset");
-	mlp::PredictArgs pa{};
-	pa.net = surrogate_net(s);
-	pa.xyz = nullptr;
-	pa.count = nbricks;
-	pa.out = nullptr;
-	pa.gx = g->x; pa.gy = g->y; pa.gz = g->z;
-	pa.dx = g->dx; pa.dy = g->dy; pa.dz = g->dz;
-	pa.cbx = ma.cbx; pa.cby = ma.cby; pa.cbz = ma.cbz;
-	pa.radius = 4.0f * std::sqrt(g->dx * g->dx + g->dy * g->dy + g->dz * g->dz);
-	pa.threshold = surrogate_threshold(s, pa.radius);
-	*threshold = pa.threshold;
-	pa.flags = m->brick_flags.as<uint8_t>();
-	if (!(pa.threshold < pa.net.clamp))
-	{
-		// The clamped fit cannot certify a brick this large: keep them all.
-		TG_CHECK_HIP(hipMemsetAsync(m->brick_flags.ptr, 1, nbricks, m->stream), "flag fill");
-	}
+	const float radius = 4.0f * std::sqrt(g->dx * g->dx + g->dy * g->dy + g->dz * g->dz);
+	sel.threshold = 0.0f;
+	if (!s)
+		TG_CHECK_HIP(hipMemsetAsync(m->brick_flags.ptr, 0, nbricks, m->stream), "flag reset");
 	else
-		TG_CHECK_HIP(mlp::launch_predict(pa, m->stream), "brick prediction launch");
+	{
+		mlp::PredictArgs pa{};
+		pa.net = surrogate_net(s);
+		pa.xyz = nullptr;
+		pa.count = nbricks;
+		pa.out = nullptr;
+		pa.gx = g->x; pa.gy = g->y; pa.gz = g->z;
+		pa.dx = g->dx; pa.dy = g->dy; pa.dz = g->dz;
+		pa.cbx = ma.cbx; pa.cby = ma.cby; pa.cbz = ma.cbz;
+		pa.radius = radius;
+		pa.threshold = surrogate_threshold(s, radius);
+		pa.flags = m->brick_flags.as<uint8_t>();
+		sel.threshold = pa.threshold;
+		if (!(pa.threshold < pa.net.clamp))
+		{
+			// The clamped fit cannot certify a brick this large: keep them all.
+			TG_CHECK_HIP(hipMemsetAsync(m->brick_flags.ptr, 1, nbricks, m->stream), "flag fill");
+		}
+		else
+			TG_CHECK_HIP(mlp::launch_predict(pa, m->stream), "brick prediction launch");
+	}
+	if (certify)
+	{
+		const bool blocks = m->blocks.valid;
+		CertifyArgs ca{};
+		ca.prog = static_cast<const Instr*>(p->device.ptr);
+		ca.pool = blocks ? m->blocks.pool.as<const Entry>() : nullptr;
+		ca.desc = blocks ? m->blocks.desc.as<const BlockDesc>() : nullptr;
+		ca.whole = p->entries.as<const Entry>();
+		ca.n_whole = uint32_t(p->decoded.entries.size());
+		ca.whole_flops = uint32_t(p->decoded.flops);
+		ca.gx = g->x; ca.gy = g->y; ca.gz = g->z;
+		ca.dx = g->dx; ca.dy = g->dy; ca.dz = g->dz;
+		ca.bx = bx; ca.by = by; ca.bz = bz;
+		ca.cbx = ma.cbx; ca.cby = ma.cby; ca.cbz = ma.cbz;
+		ca.radius = radius;
+		ca.flags = m->brick_flags.as<uint8_t>();
+		ca.counters = ctr + 2;
+		ca.slots = std::max<uint32_t>(1, std::max(blocks ? m->blocks.max_stack : 0u, p->decoded.max_stack) - 1);
+		hipError_t e = (flags & TG_FLAG_FAST) ? fast::launch_certify(ca, m->stream) : exact::launch_certify(ca, m->stream);
+		if (e != hipSuccess) return hip_fail(e, "brick certificate launch");
+	}
 	hipLaunchKernelGGL(k_tiles_from_bricks, dim3(grid_for(nbricks)), dim3(256), 0, m->stream,
 		m->brick_flags.as<const uint8_t>(), ma.cbx, ma.cby, ma.cbz, ma.tx, ma.ty, ma.tz, bx, by,
 		m->tile_mask.as<unsigned long long>(), ctr);
@@ -356,25 +403,29 @@ int select_tiles(tg_mesher* m, const tg_grid* g, const tg_surrogate* s, bool til
 		hipLaunchKernelGGL(k_list_blocks, dim3(grid_for(nblocks)), dim3(256), 0, m->stream,
 			m->tile_mask.as<const unsigned long long>(), uint32_t(nblocks), m->block_list.as<uint32_t>(), ctr + 1);
 	TG_CHECK_HIP(hipGetLastError(), "list launch");
-	unsigned long long c[2] = { 0, 0 };
-	TG_CHECK_HIP(hipMemcpyAsync(c, ctr, 16, hipMemcpyDeviceToHost, m->stream), "count read");
+	unsigned long long c[4] = { 0, 0, 0, 0 };
+	TG_CHECK_HIP(hipMemcpyAsync(c, ctr, sizeof(c), hipMemcpyDeviceToHost, m->stream), "count read");
 	TG_CHECK_HIP(hipStreamSynchronize(m->stream), "tile selection");
-	*flagged = c[0];
-	*listed = uint32_t(c[1]);
+	sel.flagged = c[0];
+	sel.listed = uint32_t(c[1]);
+	sel.certified = c[2];
+	sel.cert_flops = double(c[3]);
 	m->nbricks = nbricks;
 	ma.brick_flags = m->brick_flags.as<const uint8_t>();
 	return TG_OK;
 }
 
-int run_export(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t flags, const tg_surrogate* sur)
+int run_export(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t flags, bool culled,
+	const tg_surrogate* sur)
 {
 	int rc = validate_grid(g);
 	if (rc) return rc;
-	if (sur)
+	const bool certify = culled && (!sur || (flags & TG_FLAG_CERTIFY));
+	if (culled)
 	{
 		if (!p->decoded.lipschitz)
 			return fail(TG_ERR_PROGRAM, "culled export needs a 1-Lipschitz program (no ellipsoid, no stretching scale)");
-		if (!(surrogate_threshold(sur, 0.0f) >= 0.0f))
+		if (sur && !(surrogate_threshold(sur, 0.0f) >= 0.0f))
 			return fail(TG_ERR_INVALID, "culled export: surrogate is not calibrated");
 	}
 	m->have_grid = false;
@@ -396,39 +447,36 @@ int run_export(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fla
 	ga.row_bytes = ma.row_bytes;
 	ga.slots = std::max<uint32_t>(1, p->decoded.max_stack - 1);
 
-	const bool try_blocks = !(flags & TG_FLAG_NO_PRUNE) && p->decoded.lipschitz;
 	TG_CHECK_HIP(hipEventRecord(m->ev[0], m->stream), "event");
-	uint32_t listed = 0;
-	uint64_t flagged = 0;
-	float threshold = 0.0f;
-	if (sur)
+	m->blocks.valid = false;
+	if (!(flags & TG_FLAG_NO_PRUNE) && p->decoded.lipschitz)
 	{
-		rc = select_tiles(m, g, sur, !try_blocks, &listed, &flagged, &threshold);
+		rc = prune_blocks(m, p, g, flags);
+		if (rc) return rc;
+	}
+	const bool blocks = m->blocks.valid;
+	Selection sel;
+	if (culled)
+	{
+		rc = select_tiles(m, p, g, sur, certify, !blocks, flags, sel);
 		if (rc) return rc;
 	}
 	TG_CHECK_HIP(hipEventRecord(m->ev[3], m->stream), "event");
 	uint64_t evals = 0;
 	double flops = 0.0;
-	int pruned = 0;
-	if (try_blocks)
+	if (blocks)
 	{
-		pruned = sur ? eval_pruned(m, p, g, flags, m->tile_mask.as<const uint64_t>(), m->block_list.as<const uint32_t>(),
-		                           listed, &evals, &flops)
-		             : eval_pruned(m, p, g, flags, nullptr, nullptr, 0, &evals, &flops);
-		if (pruned < 0) return pruned;
-		if (!pruned && sur)
-		{
-			// Block evaluator unavailable (LDS): list tiles for the plain one.
-			rc = select_tiles(m, g, sur, true, &listed, &flagged, &threshold);
-			if (rc) return rc;
-		}
+		rc = culled ? eval_blocks(m, p, g, flags, m->tile_mask.as<const uint64_t>(), m->block_list.as<const uint32_t>(),
+		                          sel.listed, &evals, &flops)
+		            : eval_blocks(m, p, g, flags, nullptr, nullptr, 0, &evals, &flops);
+		if (rc) return rc;
 	}
-	if (!pruned)
+	else
 	{
-		if (sur)
+		if (culled)
 		{
 			ga.tile_list = m->tile_list.as<const uint32_t>();
-			ga.ntiles = listed;
+			ga.ntiles = sel.listed;
 		}
 		evals = uint64_t(ga.ntiles) * kTileSamples;
 		flops = double(evals) * p->decoded.flops;
@@ -441,28 +489,30 @@ int run_export(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fla
 	if (rc) return rc;
 	TG_CHECK_HIP(hipEventRecord(m->ev[2], m->stream), "event");
 	TG_CHECK_HIP(hipEventSynchronize(m->ev[2]), "export");
-	float t_pred = 0, t_eval = 0, t_mesh = 0, t_all = 0;
-	hipEventElapsedTime(&t_pred, m->ev[0], m->ev[3]);
-	hipEventElapsedTime(&t_eval, m->ev[3], m->ev[1]);
-	hipEventElapsedTime(&t_mesh, m->ev[1], m->ev[2]);
-	hipEventElapsedTime(&t_all, m->ev[0], m->ev[2]);
+	float t_pre = 0, t_eval = 0, t_mesh = 0, t_all = 0;
+	TG_CHECK_HIP(hipEventElapsedTime(&t_pre, m->ev[0], m->ev[3]), "timing");
+	TG_CHECK_HIP(hipEventElapsedTime(&t_eval, m->ev[3], m->ev[1]), "timing");
+	TG_CHECK_HIP(hipEventElapsedTime(&t_mesh, m->ev[1], m->ev[2]), "timing");
+	TG_CHECK_HIP(hipEventElapsedTime(&t_all, m->ev[0], m->ev[2]), "timing");
 	tg_export_stats& s = m->stats;
 	s = tg_export_stats{};
-	s.eval_ms = t_eval;
+	s.eval_ms = t_pre + t_eval;
 	s.mesh_ms = t_mesh;
 	s.total_ms = t_all;
 	s.samples = uint64_t(ma.nx) * ma.ny * ma.nz;
 	s.tiles_evaluated = evals / kTileSamples;
 	s.exact_evals = evals;
 	s.flops = flops;
-	if (sur)
+	s.prune_ms = t_pre;
+	if (culled)
 	{
 		s.bricks_total = m->nbricks;
-		s.bricks_flagged = flagged;
-		s.predict_ms = t_pred;
-		s.cull_threshold = threshold;
+		s.bricks_flagged = sel.flagged;
+		s.bricks_certified = sel.certified;
+		s.cull_threshold = sel.threshold;
+		s.flops += sel.cert_flops;
 	}
-	m->have_grid = sur == nullptr;
+	m->have_grid = !culled;
 	return TG_OK;
 }
 
@@ -473,14 +523,14 @@ const float* tg::mesher_vertices(const tg_mesher* m) { return m->vertices.as<con
 extern "C" int tg_mesher_export(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t flags)
 {
 	if (!m || !p) return fail(TG_ERR_INVALID, "tg_mesher_export: null argument");
-	return run_export(m, p, g, flags, nullptr);
+	return run_export(m, p, g, flags, false, nullptr);
 }
 
 extern "C" int tg_mesher_export_culled(tg_mesher* m, const tg_program* p, const tg_grid* g, const tg_surrogate* s,
 	uint32_t flags)
 {
-	if (!m || !p || !s) return fail(TG_ERR_INVALID, "tg_mesher_export_culled: null argument");
-	return run_export(m, p, g, flags, s);
+	if (!m || !p) return fail(TG_ERR_INVALID, "tg_mesher_export_culled: null argument");
+	return run_export(m, p, g, flags, true, s);
 }
 
 extern "C" int tg_mesher_brick_flags(const tg_mesher* m, uint8_t* out, uint64_t capacity, uint64_t* count)
@@ -511,6 +561,74 @@ extern "C" int tg_mesher_active_bricks(tg_mesher* m, uint8_t* out, uint64_t capa
 	TG_CHECK_HIP(hipMemcpyAsync(out, m->brick_active.ptr, std::min(capacity, n), hipMemcpyDeviceToHost, m->stream),
 		"download");
 	TG_CHECK_HIP(hipStreamSynchronize(m->stream), "active bricks");
+	return TG_OK;
+}
+
+extern "C" int tg_mesher_set_tuning(tg_mesher* m, uint32_t points_per_lane, uint32_t waves)
+{
+	if (!m || !(points_per_lane == 2 || points_per_lane == 4 || points_per_lane == 8) ||
+		!(waves == 1 || waves == 2 || waves == 4))
+		return fail(TG_ERR_INVALID, "tg_mesher_set_tuning: points per lane in {2,4,8}, waves in {1,2,4}");
+	m->points_per_lane = int(points_per_lane);
+	m->waves = int(waves);
+	return TG_OK;
+}
+
+extern "C" int tg_mesher_autotune(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t flags,
+	tg_tune_result* out)
+{
+	if (!m || !p) return fail(TG_ERR_INVALID, "tg_mesher_autotune: null argument");
+	int rc = validate_grid(g);
+	if (rc) return rc;
+	if ((flags & TG_FLAG_NO_PRUNE) || !p->decoded.lipschitz)
+		return fail(TG_ERR_INVALID, "tg_mesher_autotune: tunes the pruned block evaluator only");
+	rc = layout(m, g);
+	if (rc) return rc;
+	m->have_grid = false;
+	rc = prune_blocks(m, p, g, flags);
+	if (rc) return rc;
+	if (!m->blocks.valid) return fail(TG_ERR_INVALID, "tg_mesher_autotune: program not prunable here");
+	tg_tune_result res{};
+	const int pts_c[3] = { 2, 4, 8 };
+	const int waves_c[3] = { 1, 2, 4 };
+	double best = 1e300;
+	for (int pi = 0; pi < 3; ++pi)
+		for (int wi = 0; wi < 3; ++wi)
+		{
+			m->points_per_lane = pts_c[pi];
+			m->waves = waves_c[wi];
+			double ms = 0.0;
+			for (int rep = 0; rep < 3; ++rep)
+			{
+				uint64_t evals = 0;
+				double flops = 0.0;
+				TG_CHECK_HIP(hipEventRecord(m->ev[0], m->stream), "event");
+				rc = eval_blocks(m, p, g, flags, nullptr, nullptr, 0, &evals, &flops);
+				if (rc) return rc;
+				TG_CHECK_HIP(hipEventRecord(m->ev[1], m->stream), "event");
+				TG_CHECK_HIP(hipEventSynchronize(m->ev[1]), "tuning");
+				float t = 0;
+				TG_CHECK_HIP(hipEventElapsedTime(&t, m->ev[0], m->ev[1]), "timing");
+				if (rep > 0) ms += t;  // first run warms caches
+			}
+			ms *= 0.5;
+			const int k = pi * 3 + wi;
+			res.candidate_ms[k] = ms;
+			// a candidate the LDS could not hold ran as a smaller one: record what ran
+			res.candidate_points[k] = uint32_t(m->last_points_per_lane);
+			res.candidate_waves[k] = uint32_t(m->last_waves);
+			if (ms < best)
+			{
+				best = ms;
+				res.points_per_lane = uint32_t(m->last_points_per_lane);
+				res.waves = uint32_t(m->last_waves);
+			}
+		}
+	res.candidates = 9;
+	res.eval_ms = best;
+	m->points_per_lane = int(res.points_per_lane);
+	m->waves = int(res.waves);
+	if (out) *out = res;
 	return TG_OK;
 }
 

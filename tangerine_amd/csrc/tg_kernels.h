@@ -119,6 +119,27 @@ struct BlockEvalArgs
 	int waves;                   // waves per workgroup
 };
 
+// Exact brick certificate: wave per block (4^3 cube bricks of 8^3 cubes,
+// lane = brick), the block's pruned program evaluated at every brick centre c.
+// A brick holding a sign change has |d(c)| <= radius (1-Lipschitz field) plus
+// the rounding slack; such bricks get flags[b] = 1 (never cleared).
+struct CertifyArgs
+{
+	const Instr* prog;
+	const Entry* pool;
+	const BlockDesc* desc;       // null: every block runs the whole program
+	const Entry* whole;
+	uint32_t n_whole;
+	uint32_t whole_flops;
+	float gx, gy, gz, dx, dy, dz;
+	uint32_t bx, by, bz;         // blocks per axis
+	uint32_t cbx, cby, cbz;      // bricks per axis
+	float radius;                // brick half-diagonal
+	uint8_t* flags;
+	unsigned long long* counters;  // [0] bricks newly flagged, [1] cost units executed
+	uint32_t slots;
+};
+
 // ---- sphere tracing -------------------------------------------------------------
 
 constexpr int kTraceLevels = 4;
@@ -175,6 +196,7 @@ struct GradArgs
 // fast = contracted FP32).  `p` is points per lane for the tile kernel (4 or 8).
 namespace exact {
 hipError_t launch_points(const PointArgs& a, hipStream_t s);
+hipError_t launch_certify(const CertifyArgs& a, hipStream_t s);
 hipError_t launch_tiles(const GridArgs& a, int p, hipStream_t s);
 hipError_t launch_prune_cells(const PruneArgs& a, hipStream_t s);
 hipError_t launch_eval_blocks(const BlockEvalArgs& a, int p, hipStream_t s);
@@ -183,6 +205,7 @@ hipError_t launch_gradient(const GradArgs& a, hipStream_t s);
 }
 namespace fast {
 hipError_t launch_points(const PointArgs& a, hipStream_t s);
+hipError_t launch_certify(const CertifyArgs& a, hipStream_t s);
 hipError_t launch_tiles(const GridArgs& a, int p, hipStream_t s);
 hipError_t launch_prune_cells(const PruneArgs& a, hipStream_t s);
 hipError_t launch_eval_blocks(const BlockEvalArgs& a, int p, hipStream_t s);
@@ -192,14 +215,18 @@ hipError_t launch_gradient(const GradArgs& a, hipStream_t s);
 
 // LDS layout of the block kernel (bytes), shared by launcher and kernel:
 // decision masks (2 x max_binary) and liveness (max_len) of the block program,
-// then per wave a tile program list (max_len entries) and the spill region.
+// then per wave a tile program (max_len words) and the spill region.
 __host__ __device__ inline size_t block_table_bytes(uint32_t max_binary, uint32_t max_len)
 {
 	return size_t(max_binary) * 16 + ((size_t(max_len) * 8 + 15) & ~size_t(15));
 }
+__host__ __device__ inline size_t block_list_bytes(uint32_t max_len)
+{
+	return (size_t(max_len) * 4 + 15) & ~size_t(15);
+}
 __host__ __device__ inline size_t block_wave_bytes(uint32_t slots, int p, uint32_t max_len)
 {
-	return size_t(max_len) * 16 + spill_bytes_per_wave(slots, p);
+	return block_list_bytes(max_len) + spill_bytes_per_wave(slots, p);
 }
 __host__ __device__ inline size_t block_lds_bytes(uint32_t slots, int p, int waves, uint32_t max_binary,
 	uint32_t max_len)

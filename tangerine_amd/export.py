@@ -41,14 +41,35 @@ class Mesher:
         self._cells = tuple(int(c) for c in cells)
         return self.stats()
 
-    def export_culled(self, program, surrogate, origin, step, cells, fast: bool = False, prune: bool = True) -> dict:
-        """Export that evaluates exactly only the bricks the surrogate keeps."""
+    def export_culled(self, program, surrogate, origin, step, cells, fast: bool = False, prune: bool = True,
+                      certify: bool = True) -> dict:
+        """Export that evaluates exactly only the kept bricks: those the
+        surrogate keeps and, with `certify` (always when surrogate is None),
+        every skipped brick whose exact centre value allows a sign change."""
         g = _lib.grid_struct(origin, step, cells)
-        flags = (_lib.TG_FLAG_FAST if fast else 0) | (0 if prune else _lib.TG_FLAG_NO_PRUNE)
-        check(self.lib.tg_mesher_export_culled(self.h, program.handle, C.byref(g), surrogate.h, flags),
+        flags = ((_lib.TG_FLAG_FAST if fast else 0) | (0 if prune else _lib.TG_FLAG_NO_PRUNE) |
+                 (_lib.TG_FLAG_CERTIFY if certify else 0))
+        check(self.lib.tg_mesher_export_culled(self.h, program.handle, C.byref(g),
+                                               None if surrogate is None else surrogate.h, flags),
               "tg_mesher_export_culled")
         self._cells = tuple(int(c) for c in cells)
         return self.stats()
+
+    def autotune(self, program, origin, step, cells, fast: bool = False) -> dict:
+        """Time every (points per lane, waves) configuration of the block
+        kernel on this grid and keep the fastest for later exports."""
+        g = _lib.grid_struct(origin, step, cells)
+        r = _lib.TgTuneResult()
+        check(self.lib.tg_mesher_autotune(self.h, program.handle, C.byref(g), _lib.TG_FLAG_FAST if fast else 0,
+                                          C.byref(r)), "tg_mesher_autotune")
+        return {
+            "points_per_lane": r.points_per_lane, "waves": r.waves, "eval_ms": r.eval_ms,
+            "candidates": [{"points_per_lane": r.candidate_points[i], "waves": r.candidate_waves[i],
+                            "ms": r.candidate_ms[i]} for i in range(r.candidates)],
+        }
+
+    def set_tuning(self, points_per_lane: int, waves: int) -> None:
+        check(self.lib.tg_mesher_set_tuning(self.h, int(points_per_lane), int(waves)), "tg_mesher_set_tuning")
 
     def _bricks(self, fn, name) -> np.ndarray:
         n = C.c_uint64()

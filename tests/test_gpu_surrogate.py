@@ -7,9 +7,11 @@ Bars:
     accumulation order move a BF16 rounding now and then);
   * gradients of the mean squared error: per parameter block,
     ||g_dev - g_oracle|| <= 2e-2 ||g_oracle|| and loss within 1e-4 relative;
-  * culled export: vertices and triangles byte-identical to the plain exact
-    export of the same grid, no brick holding an active cube left out, and
-    fewer tiles evaluated than the plain export.
+  * culled export (surrogate proposal + exact brick certificate, or the
+    certificate alone): vertices and triangles byte-identical to the plain
+    exact export of the same grid, no brick holding an active cube left out,
+    and fewer tiles evaluated than the plain export; the surrogate alone
+    (uncertified) is audited against the same judge.
 """
 
 import numpy as np
@@ -123,46 +125,81 @@ def test_audit_catches_a_surrogate_that_misses(gpu):
     p[-1] = 0.4
     s.set_params(p)
     s.set_margin(0.0)
-    st = m.export_culled(prog, s, LO, step, (256, 256, 256))
+    st = m.export_culled(prog, s, LO, step, (256, 256, 256), certify=False)
     kept = m.brick_flags()
     missed = int(np.count_nonzero(active.astype(bool) & ~kept.astype(bool)))
     assert missed > 0
     assert st["bricks_flagged"] == int(kept.sum()) and st["bricks_total"] == kept.size
     v, t = m.mesh()
     assert v.shape != v_plain.shape or t.shape != t_plain.shape
+    # the exact certificate puts every missed brick back
+    st = m.export_culled(prog, s, LO, step, (256, 256, 256), certify=True)
+    kept = m.brick_flags().astype(bool)
+    assert int(np.count_nonzero(active.astype(bool) & ~kept)) == 0
+    assert st["bricks_certified"] >= missed
+    v, t = m.mesh()
+    assert v.tobytes() == v_plain.tobytes() and t.tobytes() == t_plain.tobytes()
     m.close()
     s.close()
 
 
+@pytest.mark.parametrize("fast", [False, True])
+def test_certificate_only_export_is_identical(gpu, fast):
+    """No network: every brick is kept or skipped by its exact centre value."""
+    from tangerine_amd import scenes
+    from tangerine_amd.export import Mesher
+    prog = scenes.build_sdf(scenes.random_csg_spec(2, 64)).program()
+    m = Mesher()
+    origin, step, cells = scenes.grid_for(-6.0, 6.0, 200)
+    plain = m.export(prog, origin, step, cells, fast=fast)
+    v0, t0 = m.mesh()
+    active = m.active_bricks().astype(bool)
+    culled = m.export_culled(prog, None, origin, step, cells, fast=fast)
+    kept = m.brick_flags().astype(bool)
+    v1, t1 = m.mesh()
+    assert active.sum() > 100
+    assert int(np.count_nonzero(active & ~kept)) == 0
+    assert v0.tobytes() == v1.tobytes() and t0.tobytes() == t1.tobytes()
+    assert culled["tiles_evaluated"] < 0.5 * plain["tiles_evaluated"]
+    assert culled["bricks_certified"] == culled["bricks_flagged"] == kept.sum()
+    m.close()
+
+
 def test_fit_then_culled_export_is_identical(gpu):
-    """Fit on the 256-primitive tree over [-6,6]^3, then a 256^3 culled export
-    against the plain exact export of the same grid."""
+    """Fit on the 256-primitive tree over [-6,6]^3, then a 512^3 culled export
+    (brick half-diagonal 0.16) against the plain exact export of the same
+    grid: the surrogate must keep well under half of the bricks."""
     from tangerine_amd import scenes
     from tangerine_amd.export import Mesher
     from tangerine_amd.surrogate import Surrogate
     prog = scenes.build_sdf(scenes.random_csg_spec(1, 256)).program()
-    # clamp 1: the 256^3 grid's bricks have a half-diagonal of 0.325
-    s = Surrogate(LO, SIZE, sigma=2.0, clamp=1.0, seed=1)
-    fit = s.fit(prog, samples=1 << 19, calib_samples=200_000, surface_sigma=0.05, epochs=8)
+    s = Surrogate((-6.25,) * 3, 12.5, sigma=2.0, clamp=0.5, seed=1)
+    fit = s.fit(prog, samples=1 << 20, calib_samples=200_000, surface_sigma=0.05, epochs=8)
     log = s.loss_log()
     assert log.size == fit["steps"] and np.all(np.isfinite(log))
     assert log[-8:].mean() < 0.25 * log[:8].mean()
     assert 0 < fit["mean_residual"] < 0.25 and fit["mean_residual"] < fit["margin"]
     assert fit["surface_vertices"] > 1000
     m = Mesher()
-    origin, step, cells = scenes.grid_for(-6.0, 6.0, 256)
+    origin, step, cells = scenes.grid_for(-6.0, 6.0, 512)
     plain = m.export(prog, origin, step, cells)
     v0, t0 = m.mesh()
     active = m.active_bricks().astype(bool)
+    guess = m.export_culled(prog, s, origin, step, cells, certify=False)
+    guess_missed = int(np.count_nonzero(active & ~m.brick_flags().astype(bool)))
     culled = m.export_culled(prog, s, origin, step, cells)
     kept = m.brick_flags().astype(bool)
     v1, t1 = m.mesh()
+    assert culled["bricks_certified"] >= guess_missed
+    assert culled["bricks_flagged"] == guess["bricks_flagged"] + culled["bricks_certified"]
     assert int(np.count_nonzero(active & ~kept)) == 0
     assert v0.shape == v1.shape and t0.shape == t1.shape
     assert v0.tobytes() == v1.tobytes() and t0.tobytes() == t1.tobytes()
-    assert culled["cull_threshold"] < 1.0
-    assert culled["tiles_evaluated"] < plain["tiles_evaluated"]
-    print("kept bricks", culled["bricks_flagged"], "of", culled["bricks_total"], "margin", fit["margin"],
+    assert culled["cull_threshold"] < 0.5
+    assert guess["bricks_flagged"] < 0.5 * guess["bricks_total"]
+    assert culled["tiles_evaluated"] < 0.6 * plain["tiles_evaluated"]
+    print("kept bricks", culled["bricks_flagged"], "of", culled["bricks_total"], "certified", culled["bricks_certified"],
+          "surrogate alone missed", guess_missed, "threshold", culled["cull_threshold"],
           "tiles", culled["tiles_evaluated"], "of", plain["tiles_evaluated"])
     m.close()
     s.close()
