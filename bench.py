@@ -181,6 +181,44 @@ def surrogate_leg(mesher, prog, wl, origin, step, cells, args) -> dict:
     }
 
 
+def batch_leg(args) -> dict:
+    """64 seeded trees (primitive counts log-uniform in [8, 2048], resolutions
+    from {64,128,192,256,384}^3 over [-6,6]^3) exported by one persistent
+    launch; the captured evaluation launch is then replayed and timed."""
+    from tangerine_amd.batch import Batch
+    specs = scenes.batch_specs(4, 64)
+    progs = [scenes.build_sdf(spec).program() for spec, _ in specs]
+    grids = [scenes.grid_for(-6.0, 6.0, res) for _, res in specs]
+    b = Batch()
+    t0 = time.perf_counter()
+    first = b.export(progs, grids, fast=args.fast)
+    first_wall = 1000.0 * (time.perf_counter() - t0)
+    tune = None
+    if not args.no_autotune:
+        tune = b.autotune()
+        log(f"batch autotune: {tune['points_per_lane']} points/lane, {tune['waves']} waves, "
+            f"{tune['workgroups_per_cu']} workgroups/CU, {tune['eval_ms']:.2f} ms")
+    for _ in range(args.warmup):
+        b.replay()
+    ms = [b.replay() for _ in range(max(args.steps, 1))]
+    st = b.stats()
+    b.close()
+    replay_ms = float(np.mean(ms))
+    out = {
+        "trees": len(progs), "primitives": [int(p.info["brushes"]) for p in progs],
+        "resolutions": [int(r) for _, r in specs], "samples": int(st["samples"]), "blocks": int(st["blocks"]),
+        "first_export_ms": first_wall, "prune_ms": first["prune_ms"], "eval_ms_first": first["eval_ms"],
+        "mesh_ms": first["mesh_ms"], "eval_ms_replay": replay_ms,
+        "evals_per_s": st["exact_evals"] / (replay_ms * 1e-3),
+        "tflops": st["flops"] / (replay_ms * 1e-3) / 1e12,
+        "autotune": None if tune is None else {k: tune[k] for k in ("points_per_lane", "waves", "workgroups_per_cu",
+                                                                      "eval_ms")},
+    }
+    log(f"batch: {out['samples']} samples in {len(progs)} trees, eval {replay_ms:.2f} ms per replay, "
+        f"{out['evals_per_s']:.3e} evals/s")
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -192,6 +230,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate-culled export leg")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default block-kernel launch")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-tree batched export leg")
     ap.add_argument("--sur-samples", type=int, default=2 * 1024 * 1024)
     ap.add_argument("--sur-epochs", type=int, default=16)
     ap.add_argument("--sur-clamp", type=float, default=0.5)
@@ -250,6 +289,7 @@ def main(argv=None) -> int:
     if not args.no_surrogate:
         culled = surrogate_leg(mesher, prog, wl, origin, step, cells, args)
     mesher.close()
+    batch = None if args.no_batch else batch_leg(args)
 
     ms_per_step = 1000.0 * wall / args.steps
     samples = (wl["cells"] + 1) ** 3
@@ -281,6 +321,7 @@ def main(argv=None) -> int:
         "mesh": {"vertices": nverts, "triangles": ntris},
         "autotune": tuning,
         "surrogate_culled": culled,
+        "batch64": batch,
         "roofline": {
             "bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,

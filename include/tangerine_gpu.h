@@ -60,6 +60,7 @@ typedef struct tg_program tg_program;
 typedef struct tg_mesher tg_mesher;
 typedef struct tg_tracer tg_tracer;
 typedef struct tg_surrogate tg_surrogate;
+typedef struct tg_batch tg_batch;
 
 /* Pinhole camera: pixel (i, j) looks along
  *   normalize(forward + sx*tan_half_x*right + sy*tan_half_y*up),
@@ -257,6 +258,50 @@ typedef struct tg_tune_result
 int tg_mesher_autotune(tg_mesher* mesher, const tg_program* program, const tg_grid* grid, uint32_t flags,
                        tg_tune_result* out);
 int tg_mesher_set_tuning(tg_mesher* mesher, uint32_t points_per_lane, uint32_t waves);
+
+/* ---- batched export (no reference counterpart; closest: one Sodapop
+ * MeshingJob per model on Scheduler workers, sodapop.cpp:214-224,
+ * scheduler.cpp:307-330) ------------------------------------------------------
+ * Exports many trees, each on its own grid: every tree is pruned per block,
+ * then all trees' blocks are evaluated by one persistent launch that takes
+ * work items (one block of one tree) from a device queue, heaviest trees
+ * first; the launch is captured in a hipGraph.  Each tree's samples and mesh
+ * stay in its own mesher (tg_batch_mesher), so tg_mesher_counts /
+ * tg_mesher_copy_mesh / tg_mesher_copy_grid read them.  Programs must be
+ * 1-Lipschitz and stay alive while the batch is replayed. */
+typedef struct tg_batch_stats
+{
+	double prune_ms;           /* layout and per-tree block pruning (host clock) */
+	double eval_ms;            /* the persistent launch (graph replay, HIP events) */
+	double mesh_ms;            /* surface nets of every tree (host clock) */
+	double total_ms;
+	uint64_t trees;
+	uint64_t blocks;           /* queue items */
+	uint64_t samples;
+	uint64_t exact_evals;
+	double flops;
+	uint32_t points_per_lane, waves, workgroups;
+} tg_batch_stats;
+typedef struct tg_batch_tune_result
+{
+	uint32_t points_per_lane, waves, workgroups_per_cu;  /* chosen */
+	double eval_ms;
+	uint32_t candidates;
+	uint32_t candidate_points[18], candidate_waves[18], candidate_workgroups_per_cu[18];
+	double candidate_ms[18];
+} tg_batch_tune_result;
+int tg_batch_create(tg_batch** out);
+void tg_batch_destroy(tg_batch* batch);
+int tg_batch_export(tg_batch* batch, const tg_program* const* programs, const tg_grid* grids, uint32_t count,
+                    uint32_t flags);
+/* Replays the captured evaluation launch of the last export (same trees). */
+int tg_batch_replay(tg_batch* batch, double* eval_ms);
+/* Times points per lane {2,4,8} x waves {2,4} x persistent workgroups per CU
+ * {1,2,4} on the last exported batch and keeps the fastest. */
+int tg_batch_autotune(tg_batch* batch, tg_batch_tune_result* out);
+int tg_batch_set_tuning(tg_batch* batch, uint32_t points_per_lane, uint32_t waves, uint32_t workgroups_per_cu);
+tg_mesher* tg_batch_mesher(tg_batch* batch, uint32_t index);
+int tg_batch_stats_get(const tg_batch* batch, tg_batch_stats* out);
 
 /* ---- learned brick culling (no reference counterpart) ----------------------
  * A Fourier-feature MLP (3 -> 96 features, sigma-scaled Gaussian frequencies

@@ -55,6 +55,18 @@ class TgTuneResult(C.Structure):
                 ("candidate_points", _u32 * 9), ("candidate_waves", _u32 * 9), ("candidate_ms", C.c_double * 9)]
 
 
+class TgBatchStats(C.Structure):
+    _fields_ = [("prune_ms", C.c_double), ("eval_ms", C.c_double), ("mesh_ms", C.c_double), ("total_ms", C.c_double),
+                ("trees", _u64), ("blocks", _u64), ("samples", _u64), ("exact_evals", _u64), ("flops", C.c_double),
+                ("points_per_lane", _u32), ("waves", _u32), ("workgroups", _u32)]
+
+
+class TgBatchTuneResult(C.Structure):
+    _fields_ = [("points_per_lane", _u32), ("waves", _u32), ("workgroups_per_cu", _u32), ("eval_ms", C.c_double),
+                ("candidates", _u32), ("candidate_points", _u32 * 18), ("candidate_waves", _u32 * 18),
+                ("candidate_workgroups_per_cu", _u32 * 18), ("candidate_ms", C.c_double * 18)]
+
+
 class TgCamera(C.Structure):
     _fields_ = [("eye", _f * 3), ("forward", _f * 3), ("right", _f * 3), ("up", _f * 3),
                 ("tan_half_x", _f), ("tan_half_y", _f), ("width", _u32), ("height", _u32)]
@@ -124,6 +136,14 @@ SIGNATURES = {
     "tg_mesher_stats": (C.c_int, [_p, C.POINTER(TgExportStats)]),
     "tg_mesher_autotune": (C.c_int, [_p, _p, C.POINTER(TgGrid), _u32, C.POINTER(TgTuneResult)]),
     "tg_mesher_set_tuning": (C.c_int, [_p, _u32, _u32]),
+    "tg_batch_create": (C.c_int, [C.POINTER(_p)]),
+    "tg_batch_destroy": (None, [_p]),
+    "tg_batch_export": (C.c_int, [_p, C.POINTER(_p), C.POINTER(TgGrid), _u32, _u32]),
+    "tg_batch_replay": (C.c_int, [_p, C.POINTER(C.c_double)]),
+    "tg_batch_autotune": (C.c_int, [_p, C.POINTER(TgBatchTuneResult)]),
+    "tg_batch_set_tuning": (C.c_int, [_p, _u32, _u32, _u32]),
+    "tg_batch_mesher": (_p, [_p, _u32]),
+    "tg_batch_stats_get": (C.c_int, [_p, C.POINTER(TgBatchStats)]),
     "tg_mesher_export_culled": (C.c_int, [_p, _p, C.POINTER(TgGrid), _p, _u32]),
     "tg_mesher_brick_flags": (C.c_int, [_p, C.POINTER(C.c_uint8), _u64, _u64p]),
     "tg_mesher_active_bricks": (C.c_int, [_p, C.POINTER(C.c_uint8), _u64, _u64p]),

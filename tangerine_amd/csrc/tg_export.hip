@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <vector>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -218,19 +220,11 @@ int prune_blocks(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t f
 	return prune_cells(m->stream, p, origin, cell, radius, counts, (flags & TG_FLAG_FAST) != 0, m->blocks);
 }
 
-// Pruned evaluation (k_eval_blocks) of the listed blocks' masked tiles, or of
-// every tile.  Needs m->blocks.valid.
-int eval_blocks(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t flags, const uint64_t* tile_mask,
-	const uint32_t* block_list, uint32_t n_listed, uint64_t* evals, double* flops)
+// Block-kernel arguments of one grid (after prune_blocks).
+BlockEvalArgs block_args(tg_mesher* m, const tg_program* p, const tg_grid* g, const uint64_t* tile_mask,
+	const uint32_t* block_list, uint32_t n_listed, unsigned long long* work)
 {
-	MeshArgs& ma = m->args;
-	const uint32_t counts[3] = { (ma.tx + kBlockTiles - 1) / kBlockTiles, (ma.ty + kBlockTiles - 1) / kBlockTiles,
-	                             (ma.tz + kBlockTiles - 1) / kBlockTiles };
-	int rc = m->counters.ensure(64);
-	if (rc) return rc;
-	unsigned long long* work = m->counters.as<unsigned long long>();
-	TG_CHECK_HIP(hipMemsetAsync(m->counters.ptr, 0, 64, m->stream), "counter reset");
-
+	const MeshArgs& ma = m->args;
 	BlockEvalArgs ba{};
 	ba.prog = static_cast<const Instr*>(p->device.ptr);
 	ba.pool = m->blocks.pool.as<const Entry>();
@@ -242,7 +236,9 @@ int eval_blocks(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fl
 	ba.dx = g->dx; ba.dy = g->dy; ba.dz = g->dz;
 	ba.nx = ma.nx; ba.ny = ma.ny; ba.nz = ma.nz;
 	ba.tx = ma.tx; ba.ty = ma.ty; ba.tz = ma.tz;
-	ba.bx = counts[0]; ba.by = counts[1]; ba.bz = counts[2];
+	ba.bx = (ma.tx + kBlockTiles - 1) / kBlockTiles;
+	ba.by = (ma.ty + kBlockTiles - 1) / kBlockTiles;
+	ba.bz = (ma.tz + kBlockTiles - 1) / kBlockTiles;
 	ba.tile_slot = ma.tile_slot;
 	ba.out_pool = m->pool.as<float>();
 	ba.signs = m->signs.as<uint8_t>();
@@ -256,6 +252,19 @@ int eval_blocks(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fl
 	ba.max_binary = m->blocks.max_binary;
 	ba.max_len = m->blocks.max_len;
 	ba.waves = m->waves;
+	return ba;
+}
+
+// Pruned evaluation (k_eval_blocks) of the listed blocks' masked tiles, or of
+// every tile.  Needs m->blocks.valid.
+int eval_blocks(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t flags, const uint64_t* tile_mask,
+	const uint32_t* block_list, uint32_t n_listed, uint64_t* evals, double* flops)
+{
+	int rc = m->counters.ensure(64);
+	if (rc) return rc;
+	unsigned long long* work = m->counters.as<unsigned long long>();
+	TG_CHECK_HIP(hipMemsetAsync(m->counters.ptr, 0, 64, m->stream), "counter reset");
+	BlockEvalArgs ba = block_args(m, p, g, tile_mask, block_list, n_listed, work);
 	int pts = m->points_per_lane;
 	// Fit the LDS: fewer points per lane first, then fewer waves.
 	while (pts > 2 && block_lds_bytes(ba.slots, pts, ba.waves, ba.max_binary, ba.max_len) > kMaxLds) pts >>= 1;
@@ -301,19 +310,20 @@ extern "C" int tg_mesher_create(tg_mesher** out)
 extern "C" void tg_mesher_destroy(tg_mesher* m)
 {
 	if (!m) return;
-	if (m->stream) hipStreamSynchronize(m->stream);
+	if (m->stream) (void)hipStreamSynchronize(m->stream);
 	for (DevBuf* b : { &m->pool, &m->signs, &m->active, &m->word_count, &m->word_base, &m->quad_count, &m->quad_base,
 	                   &m->vertices, &m->triangles, &m->scan_tmp, &m->dense, &m->counters, &m->brick_flags,
 	                   &m->brick_active, &m->tile_mask, &m->block_list, &m->tile_list })
 		b->release();
 	m->blocks.release();
 	for (hipEvent_t e : m->ev)
-		if (e) hipEventDestroy(e);
-	if (m->stream) hipStreamDestroy(m->stream);
+		if (e) (void)hipEventDestroy(e);
+	if (m->stream) (void)hipStreamDestroy(m->stream);
 	delete m;
 }
 
 namespace {
+
 
 // Brick selection of a culled export: brick flags from the surrogate (or none)
 // plus, with the certificate, every brick whose exact centre value allows a
@@ -706,5 +716,355 @@ extern "C" int tg_mesher_stats(const tg_mesher* m, tg_export_stats* out)
 {
 	if (!m || !out) return fail(TG_ERR_INVALID, "tg_mesher_stats: null argument");
 	*out = m->stats;
+	return TG_OK;
+}
+
+// ---- batched export: many trees, one persistent launch -----------------------
+//
+// Every tree keeps its own mesher (grid layout, pruned block programs, sample
+// tiles, mesh).  The evaluation of all trees' blocks is one launch of
+// k_eval_batch over a device queue; that launch and the reset of the queue
+// head and work counters are captured in a hipGraph, replayed by
+// tg_batch_export and tg_batch_replay.  Trees are queued heaviest first
+// (pruned-program entries as the work estimate) so that the tail of the queue
+// holds small items.
+
+struct tg_batch
+{
+	hipStream_t stream = nullptr;
+	hipEvent_t ev[2] = {};
+	std::vector<tg_mesher*> meshers;
+	std::vector<const tg_program*> programs;
+	std::vector<tg_grid> grids;
+	DevBuf trees, first, head, work;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	BatchArgs args{};
+	bool fast = false;
+	int points_per_lane = 4;
+	int waves = 4;
+	int workgroups_per_cu = 2;
+	int cus = 0;
+	int last_points_per_lane = 0, last_waves = 0, last_workgroups = 0;
+	tg_batch_stats stats{};
+};
+
+namespace {
+
+void drop_graph(tg_batch* b)
+{
+	if (b->exec) (void)hipGraphExecDestroy(b->exec);
+	if (b->graph) (void)hipGraphDestroy(b->graph);
+	b->exec = nullptr;
+	b->graph = nullptr;
+}
+
+// Captures reset + persistent launch with the batch's current tuning.
+int capture_batch(tg_batch* b)
+{
+	drop_graph(b);
+	BatchArgs& ba = b->args;
+	int pts = b->points_per_lane, waves = b->waves;
+	while (pts > 2 && block_lds_bytes(ba.slots, pts, waves, ba.max_binary, ba.max_len) > kMaxLds) pts >>= 1;
+	while (waves > 1 && block_lds_bytes(ba.slots, pts, waves, ba.max_binary, ba.max_len) > kMaxLds) waves >>= 1;
+	if (block_lds_bytes(ba.slots, pts, waves, ba.max_binary, ba.max_len) > kMaxLds)
+		return fail(TG_ERR_LIMIT, "batch: block programs too long for the LDS");
+	const uint32_t wgs = uint32_t(std::max(1, b->cus * b->workgroups_per_cu));
+	const uint32_t nwork = uint32_t(2 * b->meshers.size());
+	TG_CHECK_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "capture begin");
+	hipError_t e = b->fast ? fast::launch_batch_reset(ba.head, b->work.as<unsigned long long>(), nwork, b->stream)
+	                       : exact::launch_batch_reset(ba.head, b->work.as<unsigned long long>(), nwork, b->stream);
+	if (e == hipSuccess)
+		e = b->fast ? fast::launch_batch(ba, pts, uint32_t(waves), wgs, b->stream)
+		            : exact::launch_batch(ba, pts, uint32_t(waves), wgs, b->stream);
+	hipGraph_t g = nullptr;
+	const hipError_t ee = hipStreamEndCapture(b->stream, &g);
+	if (e != hipSuccess)
+	{
+		if (g) (void)hipGraphDestroy(g);
+		return hip_fail(e, "batch capture");
+	}
+	if (ee != hipSuccess) return hip_fail(ee, "batch capture end");
+	b->graph = g;
+	TG_CHECK_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "batch graph instantiate");
+	b->last_points_per_lane = pts;
+	b->last_waves = waves;
+	b->last_workgroups = int(wgs);
+	return TG_OK;
+}
+
+int replay_batch(tg_batch* b, float* ms)
+{
+	TG_CHECK_HIP(hipEventRecord(b->ev[0], b->stream), "event");
+	TG_CHECK_HIP(hipGraphLaunch(b->exec, b->stream), "batch graph launch");
+	TG_CHECK_HIP(hipEventRecord(b->ev[1], b->stream), "event");
+	TG_CHECK_HIP(hipEventSynchronize(b->ev[1]), "batch evaluation");
+	TG_CHECK_HIP(hipEventElapsedTime(ms, b->ev[0], b->ev[1]), "timing");
+	return TG_OK;
+}
+
+}  // namespace
+
+extern "C" int tg_batch_create(tg_batch** out)
+{
+	if (!out) return fail(TG_ERR_INVALID, "tg_batch_create: null argument");
+	int rc = ensure_device();
+	if (rc) return rc;
+	auto* b = new tg_batch;
+	int dev = 0;
+	hipError_t e = hipGetDevice(&dev);
+	if (e == hipSuccess) e = hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev);
+	if (e == hipSuccess) e = hipStreamCreate(&b->stream);
+	for (int i = 0; e == hipSuccess && i < 2; ++i) e = hipEventCreate(&b->ev[i]);
+	if (e != hipSuccess)
+	{
+		tg_batch_destroy(b);
+		return hip_fail(e, "batch stream/events");
+	}
+	*out = b;
+	return TG_OK;
+}
+
+extern "C" void tg_batch_destroy(tg_batch* b)
+{
+	if (!b) return;
+	if (b->stream) (void)hipStreamSynchronize(b->stream);
+	drop_graph(b);
+	for (tg_mesher* m : b->meshers) tg_mesher_destroy(m);
+	for (DevBuf* d : { &b->trees, &b->first, &b->head, &b->work }) d->release();
+	for (hipEvent_t e : b->ev)
+		if (e) (void)hipEventDestroy(e);
+	if (b->stream) (void)hipStreamDestroy(b->stream);
+	delete b;
+}
+
+extern "C" int tg_batch_set_tuning(tg_batch* b, uint32_t points_per_lane, uint32_t waves, uint32_t workgroups_per_cu)
+{
+	if (!b || !(points_per_lane == 2 || points_per_lane == 4 || points_per_lane == 8) ||
+		!(waves == 1 || waves == 2 || waves == 4) || workgroups_per_cu < 1 || workgroups_per_cu > 16)
+		return fail(TG_ERR_INVALID, "tg_batch_set_tuning: points per lane {2,4,8}, waves {1,2,4}, workgroups/CU 1..16");
+	b->points_per_lane = int(points_per_lane);
+	b->waves = int(waves);
+	b->workgroups_per_cu = int(workgroups_per_cu);
+	if (b->exec) return capture_batch(b);
+	return TG_OK;
+}
+
+extern "C" int tg_batch_export(tg_batch* b, const tg_program* const* programs, const tg_grid* grids, uint32_t count,
+	uint32_t flags)
+{
+	if (!b || (count && (!programs || !grids))) return fail(TG_ERR_INVALID, "tg_batch_export: null argument");
+	if (count == 0 || count > 4096) return fail(TG_ERR_INVALID, "tg_batch_export: 1..4096 trees");
+	if (flags & TG_FLAG_NO_PRUNE) return fail(TG_ERR_INVALID, "tg_batch_export: the batch evaluates pruned blocks");
+	for (uint32_t i = 0; i < count; ++i)
+	{
+		if (!programs[i]) return fail(TG_ERR_INVALID, "tg_batch_export: null program");
+		if (!programs[i]->decoded.lipschitz)
+			return fail(TG_ERR_PROGRAM, "tg_batch_export: every tree must be 1-Lipschitz (pruned evaluation)");
+		int rc = validate_grid(&grids[i]);
+		if (rc) return rc;
+	}
+	drop_graph(b);
+	b->fast = (flags & TG_FLAG_FAST) != 0;
+	while (b->meshers.size() < count)
+	{
+		tg_mesher* m = nullptr;
+		int rc = tg_mesher_create(&m);
+		if (rc) return rc;
+		b->meshers.push_back(m);
+	}
+	while (b->meshers.size() > count)
+	{
+		tg_mesher_destroy(b->meshers.back());
+		b->meshers.pop_back();
+	}
+	b->programs.assign(programs, programs + count);
+	b->grids.assign(grids, grids + count);
+	auto t0 = std::chrono::steady_clock::now();
+	int rc = b->work.ensure(size_t(count) * 16);
+	if (!rc) rc = b->head.ensure(256);
+	if (!rc) rc = b->trees.ensure(size_t(count) * sizeof(BlockEvalArgs));
+	if (!rc) rc = b->first.ensure(size_t(count + 1) * 4);
+	if (rc) return rc;
+	// 1. layout and level-1 pruning per tree
+	std::vector<BlockEvalArgs> args(count);
+	std::vector<double> weight(count);
+	BatchArgs& ba = b->args;
+	ba = BatchArgs{};
+	uint64_t samples = 0;
+	for (uint32_t i = 0; i < count; ++i)
+	{
+		tg_mesher* m = b->meshers[i];
+		m->have_grid = false;
+		m->nbricks = 0;
+		rc = layout(m, &grids[i]);
+		if (!rc) rc = prune_blocks(m, programs[i], &grids[i], flags);
+		if (rc) return rc;
+		if (!m->blocks.valid) return fail(TG_ERR_LIMIT, "tg_batch_export: a tree could not be pruned");
+		args[i] = block_args(m, programs[i], &grids[i], nullptr, nullptr, 0, b->work.as<unsigned long long>() + 2 * i);
+		weight[i] = double(m->blocks.entries) + double(m->blocks.ncells);
+		ba.slots = std::max(ba.slots, args[i].slots);
+		ba.max_binary = std::max(ba.max_binary, args[i].max_binary);
+		ba.max_len = std::max(ba.max_len, args[i].max_len);
+		samples += uint64_t(m->args.nx) * m->args.ny * m->args.nz;
+	}
+	// Trees whose tables would not fit the LDS beside the others run unpruned tiles.
+	while (block_lds_bytes(ba.slots, 2, 1, ba.max_binary, ba.max_len) > kMaxLds)
+	{
+		uint32_t worst = 0;
+		for (uint32_t i = 1; i < count; ++i)
+			if (args[i].max_len > args[worst].max_len) worst = i;
+		args[worst].prune_tiles = 0;
+		args[worst].max_len = args[worst].max_binary = 0;
+		ba.max_binary = ba.max_len = 0;
+		for (uint32_t i = 0; i < count; ++i)
+		{
+			ba.max_binary = std::max(ba.max_binary, args[i].max_binary);
+			ba.max_len = std::max(ba.max_len, args[i].max_len);
+		}
+	}
+	// 2. queue order: heaviest trees first
+	std::vector<uint32_t> order(count);
+	for (uint32_t i = 0; i < count; ++i) order[i] = i;
+	std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return weight[x] > weight[y]; });
+	std::vector<BlockEvalArgs> queued(count);
+	std::vector<uint32_t> first(count + 1, 0);
+	for (uint32_t k = 0; k < count; ++k)
+	{
+		queued[k] = args[order[k]];
+		first[k + 1] = first[k] + queued[k].nblocks;
+	}
+	TG_CHECK_HIP(hipMemcpy(b->trees.ptr, queued.data(), count * sizeof(BlockEvalArgs), hipMemcpyHostToDevice), "upload");
+	TG_CHECK_HIP(hipMemcpy(b->first.ptr, first.data(), (count + 1) * 4, hipMemcpyHostToDevice), "upload");
+	ba.trees = b->trees.as<const BlockEvalArgs>();
+	ba.first = b->first.as<const uint32_t>();
+	ba.ntrees = count;
+	ba.total = first[count];
+	ba.head = b->head.as<unsigned int>();
+	auto t1 = std::chrono::steady_clock::now();
+	// 3. one persistent launch (captured), then surface nets per tree
+	rc = capture_batch(b);
+	if (rc) return rc;
+	float eval_ms = 0;
+	rc = replay_batch(b, &eval_ms);
+	if (rc) return rc;
+	std::vector<unsigned long long> wk(2 * size_t(count));
+	TG_CHECK_HIP(hipMemcpy(wk.data(), b->work.ptr, wk.size() * 8, hipMemcpyDeviceToHost), "work read");
+	auto t2 = std::chrono::steady_clock::now();
+	double mesh_ms = 0;
+	uint64_t evals = 0;
+	double flops = 0;
+	for (uint32_t k = 0; k < count; ++k)
+	{
+		const uint32_t i = order[k];
+		tg_mesher* m = b->meshers[i];
+		rc = surface_nets(m);
+		if (rc) return rc;
+		TG_CHECK_HIP(hipStreamSynchronize(m->stream), "batch mesh");
+		m->have_grid = true;
+		tg_export_stats& s = m->stats;
+		s = tg_export_stats{};
+		s.samples = uint64_t(m->args.nx) * m->args.ny * m->args.nz;
+		s.exact_evals = wk[2 * i];
+		s.tiles_evaluated = wk[2 * i] / kTileSamples;
+		s.flops = double(wk[2 * i + 1]);
+		evals += wk[2 * i];
+		flops += double(wk[2 * i + 1]);
+	}
+	auto t3 = std::chrono::steady_clock::now();
+	mesh_ms = std::chrono::duration<double, std::milli>(t3 - t2).count();
+	tg_batch_stats& st = b->stats;
+	st = tg_batch_stats{};
+	st.prune_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+	st.eval_ms = eval_ms;
+	st.mesh_ms = mesh_ms;
+	st.total_ms = std::chrono::duration<double, std::milli>(t3 - t0).count();
+	st.trees = count;
+	st.blocks = ba.total;
+	st.samples = samples;
+	st.exact_evals = evals;
+	st.flops = flops;
+	st.points_per_lane = uint32_t(b->last_points_per_lane);
+	st.waves = uint32_t(b->last_waves);
+	st.workgroups = uint32_t(b->last_workgroups);
+	return TG_OK;
+}
+
+extern "C" int tg_batch_replay(tg_batch* b, double* eval_ms)
+{
+	if (!b || !eval_ms) return fail(TG_ERR_INVALID, "tg_batch_replay: null argument");
+	if (!b->exec) return fail(TG_ERR_INVALID, "tg_batch_replay: no batch exported yet");
+	float ms = 0;
+	int rc = replay_batch(b, &ms);
+	if (rc) return rc;
+	*eval_ms = ms;
+	return TG_OK;
+}
+
+extern "C" int tg_batch_autotune(tg_batch* b, tg_batch_tune_result* out)
+{
+	if (!b) return fail(TG_ERR_INVALID, "tg_batch_autotune: null argument");
+	if (!b->exec) return fail(TG_ERR_INVALID, "tg_batch_autotune: export a batch first");
+	const int pts_c[3] = { 2, 4, 8 };
+	const int waves_c[2] = { 2, 4 };
+	const int wg_c[3] = { 1, 2, 4 };
+	tg_batch_tune_result r{};
+	double best = 1e300;
+	uint32_t k = 0;
+	for (int pi = 0; pi < 3; ++pi)
+		for (int wi = 0; wi < 2; ++wi)
+			for (int gi = 0; gi < 3; ++gi)
+			{
+				b->points_per_lane = pts_c[pi];
+				b->waves = waves_c[wi];
+				b->workgroups_per_cu = wg_c[gi];
+				int rc = capture_batch(b);
+				if (rc) return rc;
+				float t = 0, sum = 0;
+				for (int rep = 0; rep < 3; ++rep)
+				{
+					rc = replay_batch(b, &t);
+					if (rc) return rc;
+					if (rep > 0) sum += t;
+				}
+				const double ms = 0.5 * double(sum);
+				r.candidate_points[k] = uint32_t(b->last_points_per_lane);
+				r.candidate_waves[k] = uint32_t(b->last_waves);
+				r.candidate_workgroups_per_cu[k] = uint32_t(wg_c[gi]);
+				r.candidate_ms[k] = ms;
+				if (ms < best)
+				{
+					best = ms;
+					r.points_per_lane = uint32_t(b->last_points_per_lane);
+					r.waves = uint32_t(b->last_waves);
+					r.workgroups_per_cu = uint32_t(wg_c[gi]);
+				}
+				++k;
+			}
+	r.candidates = k;
+	r.eval_ms = best;
+	b->points_per_lane = int(r.points_per_lane);
+	b->waves = int(r.waves);
+	b->workgroups_per_cu = int(r.workgroups_per_cu);
+	int rc = capture_batch(b);
+	if (rc) return rc;
+	if (out) *out = r;
+	return TG_OK;
+}
+
+extern "C" tg_mesher* tg_batch_mesher(tg_batch* b, uint32_t index)
+{
+	if (!b || index >= b->meshers.size())
+	{
+		fail(TG_ERR_INVALID, "tg_batch_mesher: index out of range");
+		return nullptr;
+	}
+	return b->meshers[index];
+}
+
+extern "C" int tg_batch_stats_get(const tg_batch* b, tg_batch_stats* out)
+{
+	if (!b || !out) return fail(TG_ERR_INVALID, "tg_batch_stats_get: null argument");
+	*out = b->stats;
 	return TG_OK;
 }

@@ -119,6 +119,18 @@ struct BlockEvalArgs
 	int waves;                   // waves per workgroup
 };
 
+// Persistent batched evaluation of many trees' blocks (one queue item = one
+// block of one tree).
+struct BatchArgs
+{
+	const BlockEvalArgs* trees;  // per tree (device array)
+	const uint32_t* first;       // [ntrees + 1] first queue item of each tree
+	uint32_t ntrees;
+	uint32_t total;              // queue items
+	unsigned int* head;          // queue head (reset to 0 before each launch)
+	uint32_t slots, max_binary, max_len;  // LDS layout: the largest over the trees
+};
+
 // Exact brick certificate: wave per block (4^3 cube bricks of 8^3 cubes,
 // lane = brick), the block's pruned program evaluated at every brick centre c.
 // A brick holding a sign change has |d(c)| <= radius (1-Lipschitz field) plus
@@ -196,6 +208,8 @@ struct GradArgs
 // fast = contracted FP32).  `p` is points per lane for the tile kernel (4 or 8).
 namespace exact {
 hipError_t launch_points(const PointArgs& a, hipStream_t s);
+hipError_t launch_batch(const BatchArgs& b, int p, uint32_t waves, uint32_t workgroups, hipStream_t s);
+hipError_t launch_batch_reset(unsigned int* head, unsigned long long* work, uint32_t nwork, hipStream_t s);
 hipError_t launch_certify(const CertifyArgs& a, hipStream_t s);
 hipError_t launch_tiles(const GridArgs& a, int p, hipStream_t s);
 hipError_t launch_prune_cells(const PruneArgs& a, hipStream_t s);
@@ -205,6 +219,8 @@ hipError_t launch_gradient(const GradArgs& a, hipStream_t s);
 }
 namespace fast {
 hipError_t launch_points(const PointArgs& a, hipStream_t s);
+hipError_t launch_batch(const BatchArgs& b, int p, uint32_t waves, uint32_t workgroups, hipStream_t s);
+hipError_t launch_batch_reset(unsigned int* head, unsigned long long* work, uint32_t nwork, hipStream_t s);
 hipError_t launch_certify(const CertifyArgs& a, hipStream_t s);
 hipError_t launch_tiles(const GridArgs& a, int p, hipStream_t s);
 hipError_t launch_prune_cells(const PruneArgs& a, hipStream_t s);
