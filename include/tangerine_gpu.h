@@ -238,6 +238,18 @@ int tg_mesher_copy_mesh(const tg_mesher* mesher, float* vertices, uint32_t* tria
 /* Samples of the last export, x fastest. */
 int tg_mesher_copy_grid(const tg_mesher* mesher, float* values);
 int tg_mesher_stats(const tg_mesher* mesher, tg_export_stats* out);
+/* Binary STL of the last export's mesh (WriteSTL, tangerine/export.cpp:111-179):
+ * 80-byte header "STL generated by TangerineCSG", uint32 triangle count, per
+ * triangle the normalised tetrahedral gradient (SDFNode::Gradient, h = 1e-4)
+ * of `program` at the triangle's centroid, the three vertices times `scale`
+ * and a zero uint16. */
+int tg_mesher_write_stl(tg_mesher* mesher, const tg_program* program, const char* path, float scale,
+                        uint32_t flags);
+/* Binary little-endian PLY (PlyHeader / WritePLY, export.cpp:181-317): per
+ * vertex x y z times `scale` and the gradient at the unscaled vertex, per face
+ * uchar 3 and three 32-bit vertex ids.  No colour (no paint on this path). */
+int tg_mesher_write_ply(tg_mesher* mesher, const tg_program* program, const char* path, float scale,
+                        uint32_t flags);
 
 /* ---- block-evaluator tuning (no reference counterpart) ----------------------
  * The pruned block kernel has two launch parameters: points per lane (2, 4

@@ -136,6 +136,8 @@ SIGNATURES = {
     "tg_mesher_stats": (C.c_int, [_p, C.POINTER(TgExportStats)]),
     "tg_mesher_autotune": (C.c_int, [_p, _p, C.POINTER(TgGrid), _u32, C.POINTER(TgTuneResult)]),
     "tg_mesher_set_tuning": (C.c_int, [_p, _u32, _u32]),
+    "tg_mesher_write_stl": (C.c_int, [_p, _p, C.c_char_p, _f, _u32]),
+    "tg_mesher_write_ply": (C.c_int, [_p, _p, C.c_char_p, _f, _u32]),
     "tg_batch_create": (C.c_int, [C.POINTER(_p)]),
     "tg_batch_destroy": (None, [_p]),
     "tg_batch_export": (C.c_int, [_p, C.POINTER(_p), C.POINTER(TgGrid), _u32, _u32]),

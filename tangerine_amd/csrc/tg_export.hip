@@ -9,6 +9,7 @@
 #include <chrono>
 #include <vector>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <string>
 
@@ -716,6 +717,156 @@ extern "C" int tg_mesher_stats(const tg_mesher* m, tg_export_stats* out)
 {
 	if (!m || !out) return fail(TG_ERR_INVALID, "tg_mesher_stats: null argument");
 	*out = m->stats;
+	return TG_OK;
+}
+
+// ---- STL / PLY writers (tangerine/export.cpp:111-179, 181-317) -----------------
+
+namespace {
+
+// Triangle centroids ((a + b) + c) / 3, the glm order of export.cpp:134.
+__global__ void __launch_bounds__(256) k_centroids(const float* v, const uint32_t* tri, uint64_t n, float* out)
+{
+	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (t >= n) return;
+	const uint32_t a = tri[3 * t], b = tri[3 * t + 1], c = tri[3 * t + 2];
+	for (int k = 0; k < 3; ++k) out[3 * t + k] = ((v[3 * a + k] + v[3 * b + k]) + v[3 * c + k]) / 3.0f;
+}
+
+// Normalised tetrahedral gradients (SDFNode::Gradient, h = 1e-4) of the
+// program at `n` device points; result copied to the host.
+int gradients_at(tg_mesher* m, const tg_program* p, const float* dev_xyz, uint64_t n, uint32_t flags,
+	std::vector<float>& out)
+{
+	out.assign(size_t(n) * 3, 0.0f);
+	if (n == 0) return TG_OK;
+	DevBuf dout;
+	struct Guard { DevBuf* b; ~Guard() { b->release(); } } guard{ &dout };
+	int rc = dout.ensure(n * 12);
+	if (rc) return rc;
+	GradArgs ga{};
+	ga.prog = static_cast<const Instr*>(p->device.ptr);
+	ga.xyz = dev_xyz;
+	ga.out = dout.as<float>();
+	ga.count = n;
+	ga.h = 1.0e-4f;
+	ga.tetra = 1;
+	ga.slots = std::max<uint32_t>(1, p->decoded.max_stack - 1);
+	hipError_t e = (flags & TG_FLAG_FAST) ? fast::launch_gradient(ga, m->stream) : exact::launch_gradient(ga, m->stream);
+	if (e != hipSuccess) return hip_fail(e, "gradient launch");
+	TG_CHECK_HIP(hipMemcpyAsync(out.data(), dout.ptr, n * 12, hipMemcpyDeviceToHost, m->stream), "normal download");
+	TG_CHECK_HIP(hipStreamSynchronize(m->stream), "normals");
+	return TG_OK;
+}
+
+struct File
+{
+	std::FILE* f = nullptr;
+	~File()
+	{
+		if (f) std::fclose(f);
+	}
+};
+
+}  // namespace
+
+extern "C" int tg_mesher_write_stl(tg_mesher* m, const tg_program* p, const char* path, float scale, uint32_t flags)
+{
+	if (!m || !p || !path) return fail(TG_ERR_INVALID, "tg_mesher_write_stl: null argument");
+	const uint64_t nt = m->ntris, nv = m->nverts;
+	if (nt >= (1ull << 32)) return fail(TG_ERR_LIMIT, "tg_mesher_write_stl: more than 2^32-1 triangles");
+	std::vector<float> verts(size_t(nv) * 3), normals;
+	std::vector<uint32_t> tris(size_t(nt) * 3);
+	int rc = tg_mesher_copy_mesh(m, verts.data(), tris.data());
+	if (rc) return rc;
+	if (nt)
+	{
+		DevBuf cen;
+		struct Guard { DevBuf* b; ~Guard() { b->release(); } } guard{ &cen };
+		rc = cen.ensure(nt * 12);
+		if (rc) return rc;
+		hipLaunchKernelGGL(k_centroids, dim3(grid_for(nt)), dim3(256), 0, m->stream, m->vertices.as<const float>(),
+			m->triangles.as<const uint32_t>(), nt, cen.as<float>());
+		TG_CHECK_HIP(hipGetLastError(), "centroid launch");
+		rc = gradients_at(m, p, cen.as<const float>(), nt, flags, normals);
+		if (rc) return rc;
+	}
+	for (float& v : verts) v *= scale;
+	File out;
+	out.f = std::fopen(path, "wb");
+	if (!out.f) return fail(TG_ERR_INVALID, std::string("tg_mesher_write_stl: cannot open ") + path);
+	char header[80] = {};
+	const char* text = "STL generated by TangerineCSG";
+	std::memcpy(header, text, std::strlen(text));
+	const uint32_t count = uint32_t(nt);
+	bool ok = std::fwrite(header, 1, 80, out.f) == 80 && std::fwrite(&count, 4, 1, out.f) == 1;
+	std::vector<unsigned char> buf;
+	buf.reserve(50 * 65536);
+	for (uint64_t t = 0; ok && t < nt; ++t)
+	{
+		unsigned char rec[50];
+		std::memcpy(rec, &normals[3 * t], 12);
+		for (int k = 0; k < 3; ++k) std::memcpy(rec + 12 + 12 * k, &verts[3 * size_t(tris[3 * t + k])], 12);
+		rec[48] = rec[49] = 0;
+		buf.insert(buf.end(), rec, rec + 50);
+		if (buf.size() >= 50 * 65536 || t + 1 == nt)
+		{
+			ok = std::fwrite(buf.data(), 1, buf.size(), out.f) == buf.size();
+			buf.clear();
+		}
+	}
+	// The reference pads (84 + 100 t) % 4 bytes, which is always zero.
+	if (!ok) return fail(TG_ERR_INVALID, std::string("tg_mesher_write_stl: write failed: ") + path);
+	return TG_OK;
+}
+
+extern "C" int tg_mesher_write_ply(tg_mesher* m, const tg_program* p, const char* path, float scale, uint32_t flags)
+{
+	if (!m || !p || !path) return fail(TG_ERR_INVALID, "tg_mesher_write_ply: null argument");
+	const uint64_t nt = m->ntris, nv = m->nverts;
+	std::vector<float> verts(size_t(nv) * 3), normals;
+	std::vector<uint32_t> tris(size_t(nt) * 3);
+	int rc = tg_mesher_copy_mesh(m, verts.data(), tris.data());
+	if (rc) return rc;
+	rc = gradients_at(m, p, m->vertices.as<const float>(), nv, flags, normals);
+	if (rc) return rc;
+	std::string header = "ply\nformat binary_little_endian 1.0\ncomment Created by Tangerine\nelement vertex " +
+	                     std::to_string(nv) +
+	                     "\nproperty float x\nproperty float y\nproperty float z\nproperty float nx\nproperty float "
+	                     "ny\nproperty float nz\n";
+	if (nt > 0) header += "element face " + std::to_string(nt) + "\nproperty list uchar uint vertex_indices\n";
+	header += "end_header\n";
+	File out;
+	out.f = std::fopen(path, "wb");
+	if (!out.f) return fail(TG_ERR_INVALID, std::string("tg_mesher_write_ply: cannot open ") + path);
+	bool ok = std::fwrite(header.data(), 1, header.size(), out.f) == header.size();
+	std::vector<unsigned char> buf;
+	buf.reserve(24 * 65536);
+	for (uint64_t v = 0; ok && v < nv; ++v)
+	{
+		float rec[6] = { verts[3 * v] * scale, verts[3 * v + 1] * scale, verts[3 * v + 2] * scale, normals[3 * v],
+		                 normals[3 * v + 1], normals[3 * v + 2] };
+		const unsigned char* r = reinterpret_cast<const unsigned char*>(rec);
+		buf.insert(buf.end(), r, r + 24);
+		if (buf.size() >= 24 * 65536 || v + 1 == nv)
+		{
+			ok = std::fwrite(buf.data(), 1, buf.size(), out.f) == buf.size();
+			buf.clear();
+		}
+	}
+	for (uint64_t t = 0; ok && t < nt; ++t)
+	{
+		unsigned char rec[13];
+		rec[0] = 3;
+		std::memcpy(rec + 1, &tris[3 * t], 12);
+		buf.insert(buf.end(), rec, rec + 13);
+		if (buf.size() >= 13 * 65536 || t + 1 == nt)
+		{
+			ok = std::fwrite(buf.data(), 1, buf.size(), out.f) == buf.size();
+			buf.clear();
+		}
+	}
+	if (!ok) return fail(TG_ERR_INVALID, std::string("tg_mesher_write_ply: write failed: ") + path);
 	return TG_OK;
 }
 

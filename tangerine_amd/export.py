@@ -112,6 +112,16 @@ class Mesher:
               "tg_mesher_copy_mesh")
         return v, t
 
+    def write_stl(self, program, path, scale: float = 1.0, fast: bool = False) -> None:
+        """Binary STL of the last export with gradient normals (WriteSTL)."""
+        check(self.lib.tg_mesher_write_stl(self.h, program.handle, str(path).encode(), float(scale),
+                                           _lib.TG_FLAG_FAST if fast else 0), "tg_mesher_write_stl")
+
+    def write_ply(self, program, path, scale: float = 1.0, fast: bool = False) -> None:
+        """Binary PLY of the last export with gradient normals (WritePLY)."""
+        check(self.lib.tg_mesher_write_ply(self.h, program.handle, str(path).encode(), float(scale),
+                                           _lib.TG_FLAG_FAST if fast else 0), "tg_mesher_write_ply")
+
     def grid(self) -> np.ndarray:
         nx, ny, nz = (c + 1 for c in self._cells)
         out = np.zeros(nx * ny * nz, dtype=np.float32)
