@@ -236,7 +236,16 @@ def main(argv=None) -> int:
     ap.add_argument("--sur-clamp", type=float, default=0.5)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("TG_CPU_THREADS", "16")))
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run the device copy kernel over BYTES (calibrates the HBM byte counters) and exit")
     args = ap.parse_args(argv)
+    if args.copy_probe:
+        lib = _lib.load()
+        ms = C.c_double()
+        _lib.check(lib.tg_device_copy_probe(args.copy_probe, 3, C.byref(ms)), "tg_device_copy_probe")
+        print(json.dumps({"copy_bytes": args.copy_probe, "ms_per_copy": ms.value,
+                          "gb_per_s": 2 * args.copy_probe / (ms.value * 1e-3) / 1e9}), flush=True)
+        return 0
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; run with --gpus 1 (multi-GPU sharding is "
