@@ -33,7 +33,7 @@ struct tg_mesher
 {
 	hipStream_t stream = nullptr;
 	hipEvent_t ev[4] = {};
-	DevBuf pool, signs, active, word_count, word_base, quad_count, quad_base, vertices, triangles, scan_tmp, dense;
+	DevBuf pool, signs, active, word_count, word_base, quad_count, quad_base, vertices, triangles, scan_tmp, dense, cubes;
 	DevBuf counters;
 	DevBuf brick_flags, brick_active, tile_mask, block_list, tile_list;
 	uint64_t nbricks = 0;         // bricks of the last culled export (0: none)
@@ -91,7 +91,10 @@ int layout(tg_mesher* m, const tg_grid* g)
 	return TG_OK;
 }
 
-// Surface nets over the samples already in pool/signs.
+// Surface nets over the samples already in pool/signs: classify cube rows,
+// scan the per-word counts into vertex ids, list the active cubes, then one
+// thread per active cube places its vertex and counts and (after a scan of the
+// counts) writes its quads.
 int surface_nets(tg_mesher* m)
 {
 	MeshArgs& a = m->args;
@@ -99,14 +102,10 @@ int surface_nets(tg_mesher* m)
 	int rc = m->active.ensure(words * 8);
 	if (!rc) rc = m->word_count.ensure(words * 4);
 	if (!rc) rc = m->word_base.ensure(words * 4);
-	if (!rc) rc = m->quad_count.ensure(words * 4);
-	if (!rc) rc = m->quad_base.ensure(words * 4);
 	if (rc) return rc;
 	a.active = m->active.as<uint64_t>();
 	a.word_count = m->word_count.as<uint32_t>();
 	a.word_base = m->word_base.as<uint32_t>();
-	a.quad_count = m->quad_count.as<uint32_t>();
-	a.quad_base = m->quad_base.as<uint32_t>();
 	size_t tmp = 0;
 	TG_CHECK_HIP(exclusive_scan_u32(nullptr, nullptr, words, nullptr, tmp, m->stream), "scan sizing");
 	rc = m->scan_tmp.ensure(tmp);
@@ -120,22 +119,39 @@ int surface_nets(tg_mesher* m)
 	TG_CHECK_HIP(hipMemcpyAsync(&tail[1], a.word_count + words - 1, 4, hipMemcpyDeviceToHost, m->stream), "count copy");
 	TG_CHECK_HIP(hipStreamSynchronize(m->stream), "vertex count");
 	m->nverts = uint64_t(tail[0]) + tail[1];
-	rc = m->vertices.ensure(std::max<uint64_t>(m->nverts, 1) * 12);
+	const uint64_t nv = m->nverts;
+	if (nv >= (1ull << 31)) return fail(TG_ERR_LIMIT, "surface nets: more than 2^31 vertices");
+	rc = m->vertices.ensure(std::max<uint64_t>(nv, 1) * 12);
+	if (!rc) rc = m->cubes.ensure(std::max<uint64_t>(nv, 1) * 4);
+	if (!rc) rc = m->quad_count.ensure(std::max<uint64_t>(nv, 1) * 4);
+	if (!rc) rc = m->quad_base.ensure(std::max<uint64_t>(nv, 1) * 4);
 	if (rc) return rc;
 	a.vertices = m->vertices.as<float>();
-	TG_CHECK_HIP(mesh_vertices(a, m->stream), "vertex launch");
-	TG_CHECK_HIP(mesh_quad_count(a, m->stream), "quad count launch");
-	tmp = m->scan_tmp.bytes;
-	TG_CHECK_HIP(exclusive_scan_u32(a.quad_count, a.quad_base, words, m->scan_tmp.ptr, tmp, m->stream), "quad scan");
-	TG_CHECK_HIP(hipMemcpyAsync(&tail[0], a.quad_base + words - 1, 4, hipMemcpyDeviceToHost, m->stream), "count copy");
-	TG_CHECK_HIP(hipMemcpyAsync(&tail[1], a.quad_count + words - 1, 4, hipMemcpyDeviceToHost, m->stream), "count copy");
-	TG_CHECK_HIP(hipStreamSynchronize(m->stream), "quad count");
-	const uint64_t quads = uint64_t(tail[0]) + tail[1];
+	a.cubes = m->cubes.as<uint32_t>();
+	a.cube_quads = m->quad_count.as<uint32_t>();
+	a.cube_qbase = m->quad_base.as<uint32_t>();
+	uint64_t quads = 0;
+	if (nv)
+	{
+		TG_CHECK_HIP(mesh_list_cubes(a, m->stream), "cube list launch");
+		TG_CHECK_HIP(mesh_vertices(a, nv, m->stream), "vertex launch");
+		TG_CHECK_HIP(mesh_quad_count(a, nv, m->stream), "quad count launch");
+		tmp = 0;
+		TG_CHECK_HIP(exclusive_scan_u32(nullptr, nullptr, nv, nullptr, tmp, m->stream), "scan sizing");
+		rc = m->scan_tmp.ensure(tmp);
+		if (rc) return rc;
+		tmp = m->scan_tmp.bytes;
+		TG_CHECK_HIP(exclusive_scan_u32(a.cube_quads, a.cube_qbase, nv, m->scan_tmp.ptr, tmp, m->stream), "quad scan");
+		TG_CHECK_HIP(hipMemcpyAsync(&tail[0], a.cube_qbase + nv - 1, 4, hipMemcpyDeviceToHost, m->stream), "count copy");
+		TG_CHECK_HIP(hipMemcpyAsync(&tail[1], a.cube_quads + nv - 1, 4, hipMemcpyDeviceToHost, m->stream), "count copy");
+		TG_CHECK_HIP(hipStreamSynchronize(m->stream), "quad count");
+		quads = uint64_t(tail[0]) + tail[1];
+	}
 	m->ntris = quads * 2;
 	rc = m->triangles.ensure(std::max<uint64_t>(quads, 1) * 24);
 	if (rc) return rc;
 	a.triangles = m->triangles.as<uint32_t>();
-	TG_CHECK_HIP(mesh_quad_write(a, m->stream), "quad write launch");
+	if (quads) TG_CHECK_HIP(mesh_quad_write(a, nv, m->stream), "quad write launch");
 	return TG_OK;
 }
 
@@ -314,7 +330,7 @@ extern "C" void tg_mesher_destroy(tg_mesher* m)
 	if (m->stream) (void)hipStreamSynchronize(m->stream);
 	for (DevBuf* b : { &m->pool, &m->signs, &m->active, &m->word_count, &m->word_base, &m->quad_count, &m->quad_base,
 	                   &m->vertices, &m->triangles, &m->scan_tmp, &m->dense, &m->counters, &m->brick_flags,
-	                   &m->brick_active, &m->tile_mask, &m->block_list, &m->tile_list })
+	                   &m->brick_active, &m->tile_mask, &m->block_list, &m->tile_list, &m->cubes })
 		b->release();
 	m->blocks.release();
 	for (hipEvent_t e : m->ev)

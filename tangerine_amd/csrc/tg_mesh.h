@@ -27,15 +27,21 @@ struct MeshArgs
 	uint32_t* word_base;
 	uint32_t* quad_count;
 	uint32_t* quad_base;
+	uint32_t* cubes;              // active cubes in vertex-id order: i + sx (j + sy k)
+	uint32_t* cube_quads;         // quads per active cube, then their exclusive scan
+	uint32_t* cube_qbase;
 	// outputs
 	float* vertices;
 	uint32_t* triangles;
 };
 
 hipError_t mesh_classify(const MeshArgs& a, hipStream_t s);
-hipError_t mesh_vertices(const MeshArgs& a, hipStream_t s);
-hipError_t mesh_quad_count(const MeshArgs& a, hipStream_t s);
-hipError_t mesh_quad_write(const MeshArgs& a, hipStream_t s);
+// Thread per active cube (after the vertex-id scan): cube list, vertices,
+// quad counts and, after their scan, triangles.
+hipError_t mesh_list_cubes(const MeshArgs& a, hipStream_t s);
+hipError_t mesh_vertices(const MeshArgs& a, uint64_t nverts, hipStream_t s);
+hipError_t mesh_quad_count(const MeshArgs& a, uint64_t nverts, hipStream_t s);
+hipError_t mesh_quad_write(const MeshArgs& a, uint64_t nverts, hipStream_t s);
 hipError_t mesh_pool_to_dense(const MeshArgs& a, float* dense, uint32_t k0, uint32_t nk, hipStream_t s);
 hipError_t mesh_dense_to_pool(const MeshArgs& a, const float* dense, hipStream_t s);
 hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, void* temp, size_t& temp_bytes, hipStream_t s);

@@ -106,8 +106,55 @@ __global__ void __launch_bounds__(256) k_classify(MeshArgs a)
 	a.word_count[t] = uint32_t(__popcll(act));
 }
 
-// Thread per active word: vertex placement.
-__global__ void __launch_bounds__(256) k_vertices(MeshArgs a)
+// Vertex of active cube (i, j, k): the mean of the edge crossings, mapped to
+// world space.
+__device__ __forceinline__ void place_vertex(const MeshArgs& a, uint32_t i, uint32_t j, uint32_t k, float* v)
+{
+	const float maxx = a.gx + float(a.sx) * a.dx;
+	const float maxy = a.gy + float(a.sy) * a.dy;
+	const float maxz = a.gz + float(a.sz) * a.dz;
+	const float fsx = float(a.sx), fsy = float(a.sy), fsz = float(a.sz);
+	float s[8];
+	s[0] = pool_at(a, i, j, k);
+	s[1] = pool_at(a, i + 1, j, k);
+	s[2] = pool_at(a, i + 1, j + 1, k);
+	s[3] = pool_at(a, i, j + 1, k);
+	s[4] = pool_at(a, i, j, k + 1);
+	s[5] = pool_at(a, i + 1, j, k + 1);
+	s[6] = pool_at(a, i + 1, j + 1, k + 1);
+	s[7] = pool_at(a, i, j + 1, k + 1);
+	// corner offsets (x, y, z) and the 12 edges of the reference cube numbering
+	constexpr uint8_t cx[8] = { 0, 1, 1, 0, 0, 1, 1, 0 };
+	constexpr uint8_t cy[8] = { 0, 0, 1, 1, 0, 0, 1, 1 };
+	constexpr uint8_t cz[8] = { 0, 0, 0, 0, 1, 1, 1, 1 };
+	constexpr uint8_t e0[12] = { 0, 1, 2, 3, 4, 5, 6, 7, 0, 1, 2, 3 };
+	constexpr uint8_t e1[12] = { 1, 2, 3, 0, 5, 6, 7, 4, 4, 5, 6, 7 };
+	const float fi = float(i), fj = float(j), fk = float(k);
+	float sx_ = 0.0f, sy_ = 0.0f, sz_ = 0.0f;
+	int n = 0;
+#pragma unroll
+	for (int e = 0; e < 12; ++e)
+	{
+		const float v1 = s[e0[e]], v2 = s[e1[e]];
+		if ((v1 >= 0.0f) == (v2 >= 0.0f)) continue;
+		const float p1x = fi + float(cx[e0[e]]), p1y = fj + float(cy[e0[e]]), p1z = fk + float(cz[e0[e]]);
+		const float p2x = fi + float(cx[e1[e]]), p2y = fj + float(cy[e1[e]]), p2z = fk + float(cz[e1[e]]);
+		const float tt = (0.0f - v1) / (v2 - v1);
+		sx_ = sx_ + (p1x + tt * (p2x - p1x));
+		sy_ = sy_ + (p1y + tt * (p2y - p1y));
+		sz_ = sz_ + (p1z + tt * (p2z - p1z));
+		++n;
+	}
+	const float fn = float(n);
+	const float gcx = sx_ / fn, gcy = sy_ / fn, gcz = sz_ / fn;
+	v[0] = a.gx + (maxx - a.gx) * (gcx - 0.0f) / (fsx - 0.0f);
+	v[1] = a.gy + (maxy - a.gy) * (gcy - 0.0f) / (fsy - 0.0f);
+	v[2] = a.gz + (maxz - a.gz) * (gcz - 0.0f) / (fsz - 0.0f);
+}
+
+// Thread per cube-row word: the word's active cubes into the cube list at
+// their vertex ids.
+__global__ void __launch_bounds__(256) k_list_cubes(MeshArgs a)
 {
 	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	const uint64_t rows = uint64_t(a.sy) * a.sz;
@@ -116,56 +163,32 @@ __global__ void __launch_bounds__(256) k_vertices(MeshArgs a)
 	if (!act) return;
 	const uint32_t w = uint32_t(t % a.aw);
 	const uint64_t row = t / a.aw;
-	const uint32_t j = uint32_t(row % a.sy), k = uint32_t(row / a.sy);
+	const uint32_t base_index = w * 64 + uint32_t(row) * a.sx;  // i + sx (j + sy k) for i = 64 w
 	uint32_t id = a.word_base[t];
-	const float maxx = a.gx + float(a.sx) * a.dx;
-	const float maxy = a.gy + float(a.sy) * a.dy;
-	const float maxz = a.gz + float(a.sz) * a.dz;
-	const float fsx = float(a.sx), fsy = float(a.sy), fsz = float(a.sz);
 	while (act)
 	{
 		const int b = __ffsll((long long)act) - 1;
 		act &= act - 1;
-		const uint32_t i = w * 64 + uint32_t(b);
-		float s[8];
-		s[0] = pool_at(a, i, j, k);
-		s[1] = pool_at(a, i + 1, j, k);
-		s[2] = pool_at(a, i + 1, j + 1, k);
-		s[3] = pool_at(a, i, j + 1, k);
-		s[4] = pool_at(a, i, j, k + 1);
-		s[5] = pool_at(a, i + 1, j, k + 1);
-		s[6] = pool_at(a, i + 1, j + 1, k + 1);
-		s[7] = pool_at(a, i, j + 1, k + 1);
-		// corner offsets (x, y, z) and the 12 edges of the reference cube numbering
-		constexpr uint8_t cx[8] = { 0, 1, 1, 0, 0, 1, 1, 0 };
-		constexpr uint8_t cy[8] = { 0, 0, 1, 1, 0, 0, 1, 1 };
-		constexpr uint8_t cz[8] = { 0, 0, 0, 0, 1, 1, 1, 1 };
-		constexpr uint8_t e0[12] = { 0, 1, 2, 3, 4, 5, 6, 7, 0, 1, 2, 3 };
-		constexpr uint8_t e1[12] = { 1, 2, 3, 0, 5, 6, 7, 4, 4, 5, 6, 7 };
-		const float fi = float(i), fj = float(j), fk = float(k);
-		float sx_ = 0.0f, sy_ = 0.0f, sz_ = 0.0f;
-		int n = 0;
-#pragma unroll
-		for (int e = 0; e < 12; ++e)
-		{
-			const float v1 = s[e0[e]], v2 = s[e1[e]];
-			if ((v1 >= 0.0f) == (v2 >= 0.0f)) continue;
-			const float p1x = fi + float(cx[e0[e]]), p1y = fj + float(cy[e0[e]]), p1z = fk + float(cz[e0[e]]);
-			const float p2x = fi + float(cx[e1[e]]), p2y = fj + float(cy[e1[e]]), p2z = fk + float(cz[e1[e]]);
-			const float tt = (0.0f - v1) / (v2 - v1);
-			sx_ = sx_ + (p1x + tt * (p2x - p1x));
-			sy_ = sy_ + (p1y + tt * (p2y - p1y));
-			sz_ = sz_ + (p1z + tt * (p2z - p1z));
-			++n;
-		}
-		const float fn = float(n);
-		const float gcx = sx_ / fn, gcy = sy_ / fn, gcz = sz_ / fn;
-		float* v = a.vertices + size_t(id) * 3;
-		v[0] = a.gx + (maxx - a.gx) * (gcx - 0.0f) / (fsx - 0.0f);
-		v[1] = a.gy + (maxy - a.gy) * (gcy - 0.0f) / (fsy - 0.0f);
-		v[2] = a.gz + (maxz - a.gz) * (gcz - 0.0f) / (fsz - 0.0f);
-		++id;
+		a.cubes[id++] = base_index + uint32_t(b);
 	}
+}
+
+__device__ __forceinline__ void cube_coords(const MeshArgs& a, uint32_t c, uint32_t& i, uint32_t& j, uint32_t& k)
+{
+	i = c % a.sx;
+	const uint32_t r = c / a.sx;
+	j = r % a.sy;
+	k = r / a.sy;
+}
+
+// Thread per active cube: vertex placement.
+__global__ void __launch_bounds__(256) k_vertices(MeshArgs a, uint64_t nverts)
+{
+	const uint64_t id = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (id >= nverts) return;
+	uint32_t i, j, k;
+	cube_coords(a, a.cubes[id], i, j, k);
+	place_vertex(a, i, j, k, a.vertices + size_t(id) * 3);
 }
 
 // Quads of one active cube: writes up to 6 triangles when `out` is set and
@@ -206,45 +229,23 @@ __device__ __forceinline__ uint32_t cube_quads(const MeshArgs& a, uint32_t i, ui
 	return count;
 }
 
-__global__ void __launch_bounds__(256) k_quad_count(MeshArgs a)
+// Thread per active cube: quads it emits, then (after their scan) the quads.
+__global__ void __launch_bounds__(256) k_quad_count(MeshArgs a, uint64_t nverts)
 {
-	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-	const uint64_t rows = uint64_t(a.sy) * a.sz;
-	if (t >= rows * a.aw) return;
-	uint64_t act = a.active[t];
-	uint32_t total = 0;
-	if (act)
-	{
-		const uint32_t w = uint32_t(t % a.aw);
-		const uint64_t row = t / a.aw;
-		const uint32_t j = uint32_t(row % a.sy), k = uint32_t(row / a.sy);
-		while (act)
-		{
-			const int b = __ffsll((long long)act) - 1;
-			act &= act - 1;
-			total += cube_quads(a, w * 64 + uint32_t(b), j, k, nullptr);
-		}
-	}
-	a.quad_count[t] = total;
+	const uint64_t id = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (id >= nverts) return;
+	uint32_t i, j, k;
+	cube_coords(a, a.cubes[id], i, j, k);
+	a.cube_quads[id] = cube_quads(a, i, j, k, nullptr);
 }
 
-__global__ void __launch_bounds__(256) k_quad_write(MeshArgs a)
+__global__ void __launch_bounds__(256) k_quad_write(MeshArgs a, uint64_t nverts)
 {
-	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-	const uint64_t rows = uint64_t(a.sy) * a.sz;
-	if (t >= rows * a.aw) return;
-	uint64_t act = a.active[t];
-	if (!act) return;
-	const uint32_t w = uint32_t(t % a.aw);
-	const uint64_t row = t / a.aw;
-	const uint32_t j = uint32_t(row % a.sy), k = uint32_t(row / a.sy);
-	uint32_t* out = a.triangles + size_t(a.quad_base[t]) * 6;
-	while (act)
-	{
-		const int b = __ffsll((long long)act) - 1;
-		act &= act - 1;
-		out += size_t(cube_quads(a, w * 64 + uint32_t(b), j, k, out)) * 6;
-	}
+	const uint64_t id = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (id >= nverts) return;
+	uint32_t i, j, k;
+	cube_coords(a, a.cubes[id], i, j, k);
+	cube_quads(a, i, j, k, a.triangles + size_t(a.cube_qbase[id]) * 6);
 }
 
 // Pool tiles -> dense x-fastest grid (for copying samples back to the host).
@@ -301,24 +302,31 @@ hipError_t mesh_classify(const MeshArgs& a, hipStream_t s)
 	return hipGetLastError();
 }
 
-hipError_t mesh_vertices(const MeshArgs& a, hipStream_t s)
+hipError_t mesh_list_cubes(const MeshArgs& a, hipStream_t s)
 {
 	const uint64_t n = uint64_t(a.sy) * a.sz * a.aw;
-	hipLaunchKernelGGL(k_vertices, dim3(blocks_for(n, 256)), dim3(256), 0, s, a);
+	hipLaunchKernelGGL(k_list_cubes, dim3(blocks_for(n, 256)), dim3(256), 0, s, a);
 	return hipGetLastError();
 }
 
-hipError_t mesh_quad_count(const MeshArgs& a, hipStream_t s)
+hipError_t mesh_vertices(const MeshArgs& a, uint64_t nverts, hipStream_t s)
 {
-	const uint64_t n = uint64_t(a.sy) * a.sz * a.aw;
-	hipLaunchKernelGGL(k_quad_count, dim3(blocks_for(n, 256)), dim3(256), 0, s, a);
+	if (nverts == 0) return hipSuccess;
+	hipLaunchKernelGGL(k_vertices, dim3(blocks_for(nverts, 256)), dim3(256), 0, s, a, nverts);
 	return hipGetLastError();
 }
 
-hipError_t mesh_quad_write(const MeshArgs& a, hipStream_t s)
+hipError_t mesh_quad_count(const MeshArgs& a, uint64_t nverts, hipStream_t s)
 {
-	const uint64_t n = uint64_t(a.sy) * a.sz * a.aw;
-	hipLaunchKernelGGL(k_quad_write, dim3(blocks_for(n, 256)), dim3(256), 0, s, a);
+	if (nverts == 0) return hipSuccess;
+	hipLaunchKernelGGL(k_quad_count, dim3(blocks_for(nverts, 256)), dim3(256), 0, s, a, nverts);
+	return hipGetLastError();
+}
+
+hipError_t mesh_quad_write(const MeshArgs& a, uint64_t nverts, hipStream_t s)
+{
+	if (nverts == 0) return hipSuccess;
+	hipLaunchKernelGGL(k_quad_write, dim3(blocks_for(nverts, 256)), dim3(256), 0, s, a, nverts);
 	return hipGetLastError();
 }
 
