@@ -40,7 +40,7 @@ int upload_program(const DecodedProgram& d, tg_program** out);
 // Level-1 pruned programs of a cell grid (BlockDesc per cell + entry pool).
 struct CellPrograms
 {
-	DevBuf pool, desc, counters;
+	DevBuf pool, desc, counters, scratch;
 	uint64_t ncells = 0;
 	uint64_t entries = 0;     // pool entries in use
 	uint32_t max_stack = 0;   // over all cells
@@ -53,6 +53,7 @@ struct CellPrograms
 		pool.release();
 		desc.release();
 		counters.release();
+		scratch.release();
 	}
 };
 

@@ -89,6 +89,7 @@ struct PruneArgs
 	uint64_t capacity;      // pool entries
 	BlockDesc* desc;        // one per cell
 	unsigned long long* counters;
+	uint64_t* scratch;      // per wave: 2 n_binary + n_entries words (decision masks, liveness)
 };
 
 // Level 2 + evaluation: one workgroup per block.
@@ -249,10 +250,15 @@ __host__ __device__ inline size_t block_lds_bytes(uint32_t slots, int p, int wav
 {
 	return block_table_bytes(max_binary, max_len) + size_t(waves) * block_wave_bytes(slots, p, max_len);
 }
-// Cell pruning: decision masks, liveness, per-lane position stacks, spill.
-__host__ __device__ inline size_t prune_lds_bytes(uint32_t n_binary, uint32_t n_entries, uint32_t slots)
+// Cell pruning: per-lane position stacks and the spill region (the decision
+// masks and liveness live in global scratch, prune_scratch_words per wave).
+__host__ __device__ inline size_t prune_lds_bytes(uint32_t slots)
 {
-	return size_t(n_binary) * 16 + size_t(n_entries) * 8 + size_t(kMaxStack) * 64 * 4 + spill_bytes_per_wave(slots, 1);
+	return size_t(kMaxStack) * 64 * 4 + spill_bytes_per_wave(slots, 1);
+}
+__host__ __device__ inline size_t prune_scratch_words(uint32_t n_binary, uint32_t n_entries)
+{
+	return 2 * size_t(n_binary) + n_entries;
 }
 
 }  // namespace tg

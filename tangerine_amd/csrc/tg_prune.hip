@@ -17,11 +17,13 @@ int prune_cells(hipStream_t stream, const tg_program* p, const float origin[3], 
 	out.valid = false;
 	if (!d.lipschitz || d.entries.empty() || d.entries.size() >= 65535) return TG_OK;
 	const uint32_t whole_slots = std::max<uint32_t>(1, d.max_stack - 1);
-	if (prune_lds_bytes(d.binary_entries, uint32_t(d.entries.size()), whole_slots) > 160 * 1024) return TG_OK;
+
 	const uint64_t ncells = uint64_t(counts[0]) * counts[1] * counts[2];
 	if (ncells == 0 || ncells >= (1ull << 31)) return TG_OK;
+	const uint64_t waves = (ncells + 63) / 64;
 	int rc = out.desc.ensure(ncells * sizeof(BlockDesc));
 	if (!rc) rc = out.counters.ensure(64);
+	if (!rc) rc = out.scratch.ensure(waves * prune_scratch_words(d.binary_entries, uint32_t(d.entries.size())) * 8);
 	if (rc) return rc;
 	// First guess: a quarter of the whole program per cell, at most 1 GiB; grown
 	// to the exact total and rerun if any cell did not fit.
@@ -48,6 +50,7 @@ int prune_cells(hipStream_t stream, const tg_program* p, const float origin[3], 
 		pa.capacity = capacity;
 		pa.desc = out.desc.as<BlockDesc>();
 		pa.counters = out.counters.as<unsigned long long>();
+		pa.scratch = out.scratch.as<uint64_t>();
 		e = fast ? fast::launch_prune_cells(pa, stream) : exact::launch_prune_cells(pa, stream);
 		if (e != hipSuccess) return hip_fail(e, "cell pruning launch");
 		unsigned long long c[5] = { 0, 0, 0, 0, 0 };
