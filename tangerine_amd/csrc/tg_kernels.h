@@ -90,6 +90,7 @@ struct PruneArgs
 	BlockDesc* desc;        // one per cell
 	unsigned long long* counters;
 	uint64_t* scratch;      // per wave: 2 n_binary + n_entries words (decision masks, liveness)
+	uint32_t cells_per_wave;  // 1..64 (lanes beyond carry no cell)
 };
 
 // Level 2 + evaluation: one workgroup per block.

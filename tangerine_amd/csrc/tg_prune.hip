@@ -20,7 +20,11 @@ int prune_cells(hipStream_t stream, const tg_program* p, const float origin[3], 
 
 	const uint64_t ncells = uint64_t(counts[0]) * counts[1] * counts[2];
 	if (ncells == 0 || ncells >= (1ull << 31)) return TG_OK;
-	const uint64_t waves = (ncells + 63) / 64;
+	// Fewer cells per wave only when there are very few cells (every wave
+	// gathers the whole program, so smaller groups cost L2 traffic).
+	uint32_t cpw = 64;
+	while (cpw > 16 && (ncells + cpw - 1) / cpw < 256) cpw >>= 1;
+	const uint64_t waves = (ncells + cpw - 1) / cpw;
 	int rc = out.desc.ensure(ncells * sizeof(BlockDesc));
 	if (!rc) rc = out.counters.ensure(64);
 	if (!rc) rc = out.scratch.ensure(waves * prune_scratch_words(d.binary_entries, uint32_t(d.entries.size())) * 8);
@@ -51,6 +55,7 @@ int prune_cells(hipStream_t stream, const tg_program* p, const float origin[3], 
 		pa.desc = out.desc.as<BlockDesc>();
 		pa.counters = out.counters.as<unsigned long long>();
 		pa.scratch = out.scratch.as<uint64_t>();
+		pa.cells_per_wave = cpw;
 		e = fast ? fast::launch_prune_cells(pa, stream) : exact::launch_prune_cells(pa, stream);
 		if (e != hipSuccess) return hip_fail(e, "cell pruning launch");
 		unsigned long long c[5] = { 0, 0, 0, 0, 0 };
