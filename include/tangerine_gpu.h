@@ -119,6 +119,9 @@ typedef struct tg_export_stats
 	double prune_ms;           /* block pruning, and for a culled export brick selection (HIP events; inside eval_ms) */
 	double cull_threshold;     /* culled export: bricks with |prediction at the centre| above this were skipped */
 	uint64_t bricks_certified; /* culled export: bricks the exact certificate kept that the surrogate skipped */
+	double block_ms;           /* the pruned block-evaluation kernel alone (HIP events; 0 if it did not run) */
+	uint32_t points_per_lane;  /* its launch: points per lane and waves per workgroup */
+	uint32_t waves;
 } tg_export_stats;
 
 typedef struct tg_surrogate_stats

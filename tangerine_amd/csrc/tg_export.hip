@@ -32,7 +32,7 @@ using namespace tg;
 struct tg_mesher
 {
 	hipStream_t stream = nullptr;
-	hipEvent_t ev[4] = {};
+	hipEvent_t ev[6] = {};
 	DevBuf pool, signs, active, word_count, word_base, quad_count, quad_base, vertices, triangles, scan_tmp, dense, cubes;
 	DevBuf counters;
 	DevBuf brick_flags, brick_active, tile_mask, block_list, tile_list;
@@ -46,6 +46,7 @@ struct tg_mesher
 	int waves = 4;                // block kernel: waves per workgroup (1, 2, 4)
 	int last_points_per_lane = 0;
 	int last_waves = 0;
+	float last_block_ms = 0.0f;   // the last k_eval_blocks launch (HIP events)
 };
 
 namespace {
@@ -294,12 +295,17 @@ int eval_blocks(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fl
 	}
 	m->last_points_per_lane = pts;
 	m->last_waves = ba.waves;
+	TG_CHECK_HIP(hipEventRecord(m->ev[4], m->stream), "event");
 	hipError_t e = (flags & TG_FLAG_FAST) ? fast::launch_eval_blocks(ba, pts, m->stream)
 	                                      : exact::launch_eval_blocks(ba, pts, m->stream);
 	if (e != hipSuccess) return hip_fail(e, "block evaluation launch");
+	TG_CHECK_HIP(hipEventRecord(m->ev[5], m->stream), "event");
 	unsigned long long wk[2] = { 0, 0 };
 	TG_CHECK_HIP(hipMemcpyAsync(wk, work, 16, hipMemcpyDeviceToHost, m->stream), "work read");
 	TG_CHECK_HIP(hipStreamSynchronize(m->stream), "block evaluation");
+	float kms = 0;
+	TG_CHECK_HIP(hipEventElapsedTime(&kms, m->ev[4], m->ev[5]), "timing");
+	m->last_block_ms = kms;
 	*evals = wk[0];
 	*flops = double(wk[1]);
 	return TG_OK;
@@ -314,7 +320,7 @@ extern "C" int tg_mesher_create(tg_mesher** out)
 	if (rc) return rc;
 	auto* m = new tg_mesher;
 	hipError_t e = hipStreamCreate(&m->stream);
-	for (int i = 0; e == hipSuccess && i < 4; ++i) e = hipEventCreate(&m->ev[i]);
+	for (int i = 0; e == hipSuccess && i < 6; ++i) e = hipEventCreate(&m->ev[i]);
 	if (e != hipSuccess)
 	{
 		delete m;
@@ -531,6 +537,9 @@ int run_export(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fla
 	s.exact_evals = evals;
 	s.flops = flops;
 	s.prune_ms = t_pre;
+	s.block_ms = blocks ? m->last_block_ms : 0.0;
+	s.points_per_lane = blocks ? uint32_t(m->last_points_per_lane) : 0u;
+	s.waves = blocks ? uint32_t(m->last_waves) : 0u;
 	if (culled)
 	{
 		s.bricks_total = m->nbricks;
