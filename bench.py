@@ -231,8 +231,9 @@ def main(argv=None) -> int:
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate-culled export leg")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default block-kernel launch")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-tree batched export leg")
-    ap.add_argument("--tuning", default="", metavar="P,W",
-                    help="block-kernel launch (points per lane, waves per workgroup) instead of autotuning")
+    ap.add_argument("--tuning", default="", metavar="P,W[,L]",
+                    help="block-kernel launch (points per lane, waves per workgroup, shortest block program pruned "
+                         "per tile) instead of autotuning")
     ap.add_argument("--sur-samples", type=int, default=2 * 1024 * 1024)
     ap.add_argument("--sur-epochs", type=int, default=16)
     ap.add_argument("--sur-clamp", type=float, default=0.5)
@@ -273,15 +274,16 @@ def main(argv=None) -> int:
 
     tuning = None
     if args.tuning:
-        pts, waves = (int(v) for v in args.tuning.split(","))
-        mesher.set_tuning(pts, waves)
-        tuning = {"points_per_lane": pts, "waves": waves, "source": "--tuning"}
+        vals = [int(v) for v in args.tuning.split(",")] + [0]
+        pts, waves, min_len = vals[0], vals[1], vals[2]
+        mesher.set_tuning(pts, waves, min_len)
+        tuning = {"points_per_lane": pts, "waves": waves, "min_prune_len": min_len, "source": "--tuning"}
     elif prune and not args.no_autotune:
         t0 = time.perf_counter()
         tuning = mesher.autotune(prog, origin, step, cells, fast=args.fast)
         tuning["wall_ms"] = 1000.0 * (time.perf_counter() - t0)
-        log(f"autotune: {tuning['points_per_lane']} points/lane, {tuning['waves']} waves, "
-            f"{tuning['eval_ms']:.2f} ms ({tuning['wall_ms']:.0f} ms to tune)")
+        log(f"autotune: {tuning['points_per_lane']} points/lane, {tuning['waves']} waves, min prune length "
+            f"{tuning['min_prune_len']}, {tuning['eval_ms']:.2f} ms ({tuning['wall_ms']:.0f} ms to tune)")
 
     for i in range(args.warmup):
         st = step_once()

@@ -257,22 +257,26 @@ int tg_mesher_write_ply(tg_mesher* mesher, const tg_program* program, const char
 /* ---- block-evaluator tuning (no reference counterpart) ----------------------
  * The pruned block kernel has two launch parameters: points per lane (2, 4
  * or 8 samples interpreted together by one lane: more work per fetched
- * instruction, more registers and LDS) and waves per workgroup (1, 2, 4).
- * tg_mesher_autotune prunes `program` for `grid`, times every combination on
- * the device and keeps the fastest for later exports on this mesher. */
+ * instruction, more registers and LDS) and waves per workgroup (1, 2, 4),
+ * and one work parameter: the shortest block program still pruned per tile.
+ * tg_mesher_autotune prunes `program` for `grid`, times every launch on the
+ * device, then the threshold at the fastest launch, and keeps the best for
+ * later exports on this mesher. */
 typedef struct tg_tune_result
 {
 	uint32_t points_per_lane;     /* chosen */
 	uint32_t waves;
+	uint32_t min_prune_len;       /* chosen: shorter block programs skip per-tile pruning */
 	double eval_ms;               /* block kernel time of the chosen configuration */
 	uint32_t candidates;
 	uint32_t candidate_points[9]; /* configuration that actually ran (LDS may force a smaller one) */
 	uint32_t candidate_waves[9];
 	double candidate_ms[9];
+	double candidate_len_ms[3];   /* min_prune_len 0, 16, 48 at the chosen launch */
 } tg_tune_result;
 int tg_mesher_autotune(tg_mesher* mesher, const tg_program* program, const tg_grid* grid, uint32_t flags,
                        tg_tune_result* out);
-int tg_mesher_set_tuning(tg_mesher* mesher, uint32_t points_per_lane, uint32_t waves);
+int tg_mesher_set_tuning(tg_mesher* mesher, uint32_t points_per_lane, uint32_t waves, uint32_t min_prune_len);
 
 /* ---- batched export (no reference counterpart; closest: one Sodapop
  * MeshingJob per model on Scheduler workers, sodapop.cpp:214-224,

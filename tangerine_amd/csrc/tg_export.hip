@@ -44,6 +44,7 @@ struct tg_mesher
 	tg_export_stats stats{};
 	int points_per_lane = 4;      // block kernel: points per lane (2, 4, 8; tg_mesher_autotune)
 	int waves = 4;                // block kernel: waves per workgroup (1, 2, 4)
+	uint32_t min_prune_len = 0;   // block kernel: shorter block programs skip per-tile pruning
 	int last_points_per_lane = 0;
 	int last_waves = 0;
 	float last_block_ms = 0.0f;   // the last k_eval_blocks launch (HIP events)
@@ -270,6 +271,7 @@ BlockEvalArgs block_args(tg_mesher* m, const tg_program* p, const tg_grid* g, co
 	ba.max_binary = m->blocks.max_binary;
 	ba.max_len = m->blocks.max_len;
 	ba.waves = m->waves;
+	ba.min_prune_len = m->min_prune_len;
 	return ba;
 }
 
@@ -600,13 +602,14 @@ extern "C" int tg_mesher_active_bricks(tg_mesher* m, uint8_t* out, uint64_t capa
 	return TG_OK;
 }
 
-extern "C" int tg_mesher_set_tuning(tg_mesher* m, uint32_t points_per_lane, uint32_t waves)
+extern "C" int tg_mesher_set_tuning(tg_mesher* m, uint32_t points_per_lane, uint32_t waves, uint32_t min_prune_len)
 {
 	if (!m || !(points_per_lane == 2 || points_per_lane == 4 || points_per_lane == 8) ||
 		!(waves == 1 || waves == 2 || waves == 4))
 		return fail(TG_ERR_INVALID, "tg_mesher_set_tuning: points per lane in {2,4,8}, waves in {1,2,4}");
 	m->points_per_lane = int(points_per_lane);
 	m->waves = int(waves);
+	m->min_prune_len = min_prune_len;
 	return TG_OK;
 }
 
@@ -660,10 +663,34 @@ extern "C" int tg_mesher_autotune(tg_mesher* m, const tg_program* p, const tg_gr
 				res.waves = uint32_t(m->last_waves);
 			}
 		}
-	res.candidates = 9;
-	res.eval_ms = best;
 	m->points_per_lane = int(res.points_per_lane);
 	m->waves = int(res.waves);
+	// then the shortest block program worth pruning per tile, at that launch
+	const uint32_t len_c[3] = { 0, 16, 48 };
+	res.min_prune_len = 0;
+	for (int li = 0; li < 3; ++li)
+	{
+		m->min_prune_len = len_c[li];
+		double ms = 0.0;
+		for (int rep = 0; rep < 3; ++rep)
+		{
+			uint64_t evals = 0;
+			double flops = 0.0;
+			rc = eval_blocks(m, p, g, flags, nullptr, nullptr, 0, &evals, &flops);
+			if (rc) return rc;
+			if (rep > 0) ms += m->last_block_ms;
+		}
+		ms *= 0.5;
+		res.candidate_len_ms[li] = ms;
+		if (li == 0 || ms < best)
+		{
+			best = ms;
+			res.min_prune_len = len_c[li];
+		}
+	}
+	m->min_prune_len = res.min_prune_len;
+	res.candidates = 9;
+	res.eval_ms = best;
 	if (out) *out = res;
 	return TG_OK;
 }

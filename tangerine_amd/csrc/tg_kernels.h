@@ -112,6 +112,7 @@ struct BlockEvalArgs
 	uint32_t row_bytes;
 	uint32_t slots;              // LDS spill slots (max block stack - 1)
 	int prune_tiles;             // 0: evaluate block programs unpruned per tile
+	uint32_t min_prune_len;      // blocks with shorter programs skip the per-tile pruning
 	unsigned long long* work;    // [0] samples interpreted, [1] flops units executed
 	const Entry* whole;          // whole program, for blocks whose pruning overflowed
 	uint32_t n_whole;

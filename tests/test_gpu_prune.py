@@ -103,23 +103,24 @@ def test_deep_stack_and_partial_blocks(mesher, oracle):
 
 
 def test_every_tuning_gives_the_same_grid(mesher):
-    """Every (points per lane, waves) launch of the block kernel writes the
-    same bits, and the autotuner's choice is one of the timed candidates."""
+    """Every (points per lane, waves, shortest pruned block program) setting
+    of the block kernel writes the same bits, and the autotuner's choice is
+    one of the timed candidates."""
     from tangerine_amd import scenes
     prog = scenes.build_sdf(scenes.random_csg_spec(1, 256)).program()
     origin, step, cells = scenes.grid_for(-6.0, 6.0, 100)
     ref = None
-    for pts in (2, 4, 8):
-        for waves in (1, 2, 4):
-            mesher.set_tuning(pts, waves)
-            mesher.export(prog, origin, step, cells)
-            g = bits(mesher.grid())
-            if ref is None:
-                ref = g
-            assert np.array_equal(g, ref), (pts, waves)
+    launches = [(p, w, 0) for p in (2, 4, 8) for w in (1, 2, 4)] + [(4, 2, 16), (2, 4, 1000)]
+    for pts, waves, min_len in launches:
+        mesher.set_tuning(pts, waves, min_len)
+        mesher.export(prog, origin, step, cells)
+        g = bits(mesher.grid())
+        if ref is None:
+            ref = g
+        assert np.array_equal(g, ref), (pts, waves, min_len)
     r = mesher.autotune(prog, origin, step, cells)
     assert len(r["candidates"]) == 9 and r["eval_ms"] > 0
     assert (r["points_per_lane"], r["waves"]) in {(c["points_per_lane"], c["waves"]) for c in r["candidates"]}
     mesher.export(prog, origin, step, cells)
     assert np.array_equal(bits(mesher.grid()), ref)
-    mesher.set_tuning(4, 4)
+    mesher.set_tuning(4, 4, 0)
