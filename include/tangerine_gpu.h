@@ -254,6 +254,41 @@ int tg_mesher_write_stl(tg_mesher* mesher, const tg_program* program, const char
 int tg_mesher_write_ply(tg_mesher* mesher, const tg_program* program, const char* path, float scale,
                         uint32_t flags);
 
+/* Grid of the last export (origin, step, cells). */
+int tg_mesher_grid_get(const tg_mesher* mesher, tg_grid* out);
+
+/* ---- point-cloud export (PointCloudExportThread, export.cpp:384-480) ---------
+ * Cells of the grid from `lo` with `step`, ceil((hi - lo) / step) per axis:
+ * a cell's centre (float(i) * step + lo) + step / 2 is kept when the program's
+ * |distance| there is below the half step's length; kept centres (raster
+ * order; the threaded reference appends them in thread order) then take
+ * `refine_iterations` steps p -= d(p) * gradient(p) (tetrahedral gradient,
+ * h = 1e-4), are clamped to their cell and replace the centre when within the
+ * half step's length of it.  The points become the mesher's vertices (no
+ * triangles): tg_mesher_counts, tg_mesher_copy_mesh, and tg_mesher_write_ply
+ * writes the reference's point-cloud PLY (normals = gradients, no faces).
+ * Whole program, exact numerics unless TG_FLAG_FAST; at most 2^31 - 1 cells. */
+int tg_mesher_export_points(tg_mesher* mesher, const tg_program* program, const float lo[3], const float hi[3],
+                            const float step[3], int refine_iterations, uint32_t flags);
+
+/* ---- Sodapop meshing job (MeshingJob::NaiveSurfaceNets, sodapop.cpp:562-897) --
+ * The interactive mesher's grid (NaiveSurfaceNetsScratch, sodapop.cpp:156-177):
+ * the bounding cube of `bounds` (min xyz, max xyz: the model's SDFNode::Bounds),
+ * floor(density) samples per unit (at least 8 per axis; the reference passes
+ * 20 + the model's density push), origin moved down two steps, three more
+ * cells per axis; the samples are exact (the reference's inexact octree values
+ * differ only away from the surface, where they are clamped to +-100 and do not
+ * change the mesh).  With `shuffle`, vertices are reordered as
+ * ApplyVertexSequence does for VertexSequence::Shuffle (sodapop.cpp:337-394:
+ * std::mt19937 seeded 0, std::shuffle; libstdc++'s shuffle) and the triangle
+ * indices remapped.  Normals: the tetrahedral gradient at every final vertex
+ * (USE_GRADIENT_NORMALS, sodapop.cpp:40, 805-817), read with
+ * tg_mesher_copy_normals.  Vertices before the shuffle follow raster order
+ * (the reference's come in task order, which varies between runs). */
+int tg_mesher_export_sodapop(tg_mesher* mesher, const tg_program* program, const float bounds[6], float density,
+                             int shuffle, uint32_t flags);
+int tg_mesher_copy_normals(const tg_mesher* mesher, float* normals);
+
 /* ---- block-evaluator tuning (no reference counterpart) ----------------------
  * The pruned block kernel has two launch parameters: points per lane (2, 4
  * or 8 samples interpreted together by one lane: more work per fetched

@@ -230,6 +230,30 @@ class Oracle(_Cpu):
         m = self.lib.tgo_surface_nets(_fptr(v), _fptr(o), _fptr(s), c.ctypes.data_as(_u64p))
         return self._take_mesh(m)
 
+    def point_cloud(self, words, lo, hi, step, iterations=0) -> np.ndarray:
+        """PointCloudExportThread's points (raster order) over the whole program."""
+        w = np.ascontiguousarray(words, dtype=np.uint32)
+        f3 = [np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float32), (3,))) for v in (lo, hi, step)]
+        L = self.lib
+        L.tgo_point_cloud.restype = C.c_int64
+        L.tgo_point_cloud.argtypes = [_u32p, _u64, _fp, _fp, _fp, C.c_int, _fp, _u64]
+        n = L.tgo_point_cloud(w.ctypes.data_as(_u32p), w.size, *[_fptr(v) for v in f3], int(iterations), None, 0)
+        if n < 0:
+            raise ValueError("malformed program")
+        out = np.zeros((n, 3), dtype=np.float32)
+        L.tgo_point_cloud(w.ctypes.data_as(_u32p), w.size, *[_fptr(v) for v in f3], int(iterations), _fptr(out), n)
+        return out
+
+    def mt19937_10000th(self) -> int:
+        self.lib.tgo_mt19937_10000th.restype = C.c_uint32
+        return int(self.lib.tgo_mt19937_10000th())
+
+    def vertex_sequence(self, n: int) -> np.ndarray:
+        out = np.zeros(n, dtype=np.uint32)
+        self.lib.tgo_vertex_sequence.argtypes = [C.c_uint64, _u32p]
+        self.lib.tgo_vertex_sequence(int(n), out.ctypes.data_as(_u32p))
+        return out
+
 
 class Reference(_Cpu):
     prefix = "tgref_"
@@ -257,6 +281,16 @@ class Reference(_Cpu):
         L.tgref_mesh_evals.argtypes = [_p]
         L.tgref_threads_default.restype = C.c_int
         L.tgref_hardware_concurrency.restype = C.c_int
+
+    def point_cloud(self, h, lo, hi, step, iterations=0) -> np.ndarray:
+        f3 = [np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float32), (3,))) for v in (lo, hi, step)]
+        L = self.lib
+        L.tgref_point_cloud.restype = C.c_int64
+        L.tgref_point_cloud.argtypes = [_p, _fp, _fp, _fp, C.c_int, _fp, _u64]
+        n = L.tgref_point_cloud(h, *[_fptr(v) for v in f3], int(iterations), None, 0)
+        out = np.zeros((n, 3), dtype=np.float32)
+        L.tgref_point_cloud(h, *[_fptr(v) for v in f3], int(iterations), _fptr(out), n)
+        return out
 
     def interp_eval(self, h, points) -> np.ndarray:
         pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)

@@ -19,6 +19,8 @@
 //                      on a pool of max(hw-1,2)+1 threads (tangerine/scheduler.cpp:141-148)
 //   pool grid fill  -> Pool (tangerine/threadpool.cpp:27-40), loop shape of
 //                      tangerine/export.cpp:401-427, SDFOctree::Eval(Exact)
+//   point cloud     -> the steps of PointCloudExportThread (export.cpp:384-480;
+//                      export.cpp itself needs SDL headers) over the whole tree
 
 #include <atomic>
 #include <cmath>
@@ -258,6 +260,59 @@ int tgref_pool_grid_eval(void* O, const float* Origin, const float* Step, const 
 		}
 	});
 	return std::max(int(std::thread::hardware_concurrency()), 2);
+}
+
+// PointCloudExportThread (tangerine/export.cpp:384-480) with the octree
+// replaced by the whole tree: the distance from SDFInterpreter::Eval, the
+// refinement direction from SDFNode::Gradient, as SDFOctree::Eval and
+// SDFOctree::Gradient call them on a leaf.  Serial, so the points come in
+// raster order.  Writes up to Capacity points; returns the count.
+int64_t tgref_point_cloud(void* A, const float* Lo, const float* Hi, const float* StepIn, int RefineIterations,
+	float* Out, uint64_t Capacity)
+{
+	SDFNodeShared Node = ((Handle*)A)->Node;
+	SDFInterpreter Interp(Node);
+	const vec3 Start(Lo[0], Lo[1], Lo[2]), Stop(Hi[0], Hi[1], Hi[2]), Step(StepIn[0], StepIn[1], StepIn[2]);
+	const vec3 Half = Step / vec3(2.0);
+	const float Diagonal = length(Half);
+	const ivec3 Iterations = ivec3(ceil((Stop - Start) / Step));
+	const int Slice = Iterations.x * Iterations.y;
+	const int TotalCells = Iterations.x * Iterations.y * Iterations.z;
+	std::vector<vec3> Vertices;
+	for (int i = 0; i < TotalCells; ++i)
+	{
+		float Z = float(i / Slice) * Step.z + Start.z;
+		float Y = float((i % Slice) / Iterations.x) * Step.y + Start.y;
+		float X = float(i % Iterations.x) * Step.x + Start.x;
+		vec3 Cursor = vec3(X, Y, Z) + Half;
+		float Dist = Interp.Eval(Cursor);
+		if (abs(Dist) < Diagonal)
+		{
+			Vertices.push_back(Cursor);
+		}
+	}
+	for (vec3& Vertex : Vertices)
+	{
+		vec3 Low = Vertex - vec3(Half);
+		vec3 High = Vertex + vec3(Half);
+		vec3 Cursor = Vertex;
+		for (int r = 0; r < RefineIterations; ++r)
+		{
+			vec3 RayDir = Node->Gradient(Cursor);
+			float Dist = Interp.Eval(Cursor) * -1.0;
+			Cursor += RayDir * Dist;
+		}
+		Cursor = clamp(Cursor, Low, High);
+		if (distance(Cursor, Vertex) <= Diagonal)
+		{
+			Vertex = Cursor;
+		}
+	}
+	for (uint64_t i = 0; i < Vertices.size() && i < Capacity; ++i)
+	{
+		Out[3 * i] = Vertices[i].x; Out[3 * i + 1] = Vertices[i].y; Out[3 * i + 2] = Vertices[i].z;
+	}
+	return int64_t(Vertices.size());
 }
 
 // Serial reference surface nets over precomputed sample values (identical-grid

@@ -37,6 +37,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <random>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -727,6 +728,29 @@ H* wrap(NodeP n) { return n ? new H{ n } : nullptr; }
 AABBo sym(V3 hi) { return { { -hi.x, -hi.y, -hi.z }, hi }; }
 }
 
+// ApplyVertexSequence with VertexSequence::Shuffle (tangerine/sodapop.cpp:
+// 337-394): exchange[i] = the index vertex i moves to.  The generator is
+// std::mt19937 seeded 0 and the shuffle is the C++ library's std::shuffle
+// (libstdc++ here, as in the reference's Linux builds).
+static void vertex_sequence(uint64_t n, uint32_t* exchange)
+{
+	for (uint64_t i = 0; i < n; ++i) exchange[i] = uint32_t(i);
+	const size_t half = size_t(n) / 2;
+	const size_t mirror = (n % 2) == 0 ? half : half + 1;
+	std::vector<size_t> sequence;
+	sequence.reserve(mirror);
+	for (size_t k = 0; k < half; ++k) sequence.push_back(mirror + k);
+	std::mt19937 generator;
+	generator.seed(0);
+	std::shuffle(sequence.begin(), sequence.end(), generator);
+	if (mirror != half) sequence.push_back(half);
+	for (size_t target = 0; target < sequence.size(); ++target)
+	{
+		exchange[target] = uint32_t(sequence[target]);
+		exchange[sequence[target]] = uint32_t(target);
+	}
+}
+
 extern "C" {
 
 void* tgo_sphere(float r) { return wrap(make_brush(Sphere, r, 0, 0, sym({ r, r, r }))); }
@@ -921,39 +945,96 @@ int tgo_gradient_central_words(const uint32_t* words, uint64_t nwords, const flo
 }
 
 // SDFNode::Gradient's tetrahedral rule (sdf_evaluator.cpp:298-333) over the
-// interpreter instead of the tree walk.
+// interpreter instead of the tree walk, at one point.
+static bool gradient_tetra_words(const uint32_t* words, uint64_t nwords, V3 c, float h, V3* out)
+{
+	V2 o = { 1.0f * h, -1.0f * h };
+	V3 kxyy = { o.x, o.y, o.y }, kyyx = { o.y, o.y, o.x }, kyxy = { o.y, o.x, o.y }, kxxx = { o.x, o.x, o.x };
+	bool ok = true, k1, k2, k3, k4;
+	float e1 = interpret(words, nwords, add3(c, kxyy), &k1);
+	float e2 = interpret(words, nwords, add3(c, kyyx), &k2);
+	float e3 = interpret(words, nwords, add3(c, kyxy), &k3);
+	float e4 = interpret(words, nwords, add3(c, kxxx), &k4);
+	ok = k1 && k2 && k3 && k4;
+	V3 g = add3(add3(add3(mul3s(kxyy, e1), mul3s(kyyx, e2)), mul3s(kyxy, e3)), mul3s(kxxx, e4));
+	float l2 = dot3(g, g);
+	if (l2 == 0.0f)
+	{
+		float d0 = interpret(words, nwords, c, &k1);
+		V3 v = { interpret(words, nwords, add3(c, kxyy), &k2) - d0, interpret(words, nwords, add3(c, kyxy), &k3) - d0,
+			interpret(words, nwords, add3(c, kyyx), &k4) - d0 };
+		ok = ok && k1 && k2 && k3 && k4;
+		*out = mul3s(v, 1.0f / std::sqrt(dot3(v, v)));
+	}
+	else
+	{
+		*out = div3s(g, std::sqrt(l2));
+	}
+	return ok;
+}
+
 int tgo_gradient_tetra_words(const uint32_t* words, uint64_t nwords, const float* p, uint64_t n, float h, float* out)
 {
 	for (uint64_t i = 0; i < n; ++i)
 	{
-		V3 c = { p[3 * i], p[3 * i + 1], p[3 * i + 2] };
-		V2 o = { 1.0f * h, -1.0f * h };
-		V3 kxyy = { o.x, o.y, o.y }, kyyx = { o.y, o.y, o.x }, kyxy = { o.y, o.x, o.y }, kxxx = { o.x, o.x, o.x };
-		bool ok = true, k1, k2, k3, k4;
-		float e1 = interpret(words, nwords, add3(c, kxyy), &k1);
-		float e2 = interpret(words, nwords, add3(c, kyyx), &k2);
-		float e3 = interpret(words, nwords, add3(c, kyxy), &k3);
-		float e4 = interpret(words, nwords, add3(c, kxxx), &k4);
-		ok = k1 && k2 && k3 && k4;
-		V3 g = add3(add3(add3(mul3s(kxyy, e1), mul3s(kyyx, e2)), mul3s(kyxy, e3)), mul3s(kxxx, e4));
-		float l2 = dot3(g, g);
 		V3 r;
-		if (l2 == 0.0f)
-		{
-			float d0 = interpret(words, nwords, c, &k1);
-			V3 v = { interpret(words, nwords, add3(c, kxyy), &k2) - d0, interpret(words, nwords, add3(c, kyxy), &k3) - d0,
-				interpret(words, nwords, add3(c, kyyx), &k4) - d0 };
-			ok = ok && k1 && k2 && k3 && k4;
-			r = mul3s(v, 1.0f / std::sqrt(dot3(v, v)));
-		}
-		else
-		{
-			r = div3s(g, std::sqrt(l2));
-		}
-		if (!ok) return -1;
+		if (!gradient_tetra_words(words, nwords, { p[3 * i], p[3 * i + 1], p[3 * i + 2] }, h, &r)) return -1;
 		out[3 * i] = r.x; out[3 * i + 1] = r.y; out[3 * i + 2] = r.z;
 	}
 	return 0;
+}
+
+// PointCloudExportThread (export.cpp:384-480) over the interpreter of the
+// whole program, serially in raster order: cell centres
+// (float(i) * Step + Start) + Half with |d| < length(Half), then
+// `iterations` steps Cursor += Gradient * -d, clamp to the cell, keep if
+// within length(Half) of the centre.  Writes up to `cap` points and returns
+// how many there are, or -1 for a malformed program.
+int64_t tgo_point_cloud(const uint32_t* words, uint64_t nwords, const float* lo, const float* hi, const float* step,
+	int iterations, float* out, uint64_t cap)
+{
+	const V3 start = { lo[0], lo[1], lo[2] }, st = { step[0], step[1], step[2] };
+	const V3 half = { st.x / 2.0f, st.y / 2.0f, st.z / 2.0f };
+	const float diagonal = len3(half);
+	int64_t it[3];
+	for (int a = 0; a < 3; ++a)
+	{
+		const float c = std::ceil((hi[a] - lo[a]) / step[a]);
+		it[a] = c > 0.0f ? int64_t(c) : 0;
+	}
+	const int64_t slice = it[0] * it[1], total = slice * it[2];
+	std::vector<V3> verts;
+	for (int64_t i = 0; i < total; ++i)
+	{
+		const float z = float(i / slice) * st.z + start.z;
+		const float y = float((i % slice) / it[0]) * st.y + start.y;
+		const float x = float(i % it[0]) * st.x + start.x;
+		const V3 cursor = add3({ x, y, z }, half);
+		bool ok;
+		const float d = interpret(words, nwords, cursor, &ok);
+		if (!ok) return -1;
+		if (std::fabs(d) < diagonal) verts.push_back(cursor);
+	}
+	for (V3& v : verts)
+	{
+		const V3 low = { v.x - half.x, v.y - half.y, v.z - half.z }, high = add3(v, half);
+		V3 c = v;
+		for (int r = 0; r < iterations; ++r)
+		{
+			V3 dir;
+			bool ok = gradient_tetra_words(words, nwords, c, 1.0e-4f, &dir);
+			const float dist = float(double(interpret(words, nwords, c, &ok)) * -1.0);
+			c = add3(c, mul3s(dir, dist));
+		}
+		c = { gclamp(c.x, low.x, high.x), gclamp(c.y, low.y, high.y), gclamp(c.z, low.z, high.z) };
+		const V3 e = { v.x - c.x, v.y - c.y, v.z - c.z };
+		if (len3(e) <= diagonal) v = c;
+	}
+	for (size_t i = 0; i < verts.size() && i < cap; ++i)
+	{
+		out[3 * i] = verts[i].x; out[3 * i + 1] = verts[i].y; out[3 * i + 2] = verts[i].z;
+	}
+	return int64_t(verts.size());
 }
 
 // SDFNode::RayMarch (sdf_evaluator.cpp:336-353) on the tree.
@@ -1123,5 +1204,18 @@ void tgo_mesh_copy(void* m, float* v, uint64_t* f)
 	std::memcpy(f, o->f.data(), o->f.size() * 8);
 }
 void tgo_mesh_free(void* m) { delete (OMesh*)m; }
+
+
+// mt19937 self-check value (the standard's 10000th output for the default seed)
+// and the vertex exchange map of ApplyVertexSequence.
+uint32_t tgo_mt19937_10000th(void)
+{
+	std::mt19937 g;
+	uint32_t v = 0;
+	for (int i = 0; i < 10000; ++i) v = g();
+	return v;
+}
+
+void tgo_vertex_sequence(uint64_t n, uint32_t* exchange) { vertex_sequence(n, exchange); }
 
 }  // extern "C"

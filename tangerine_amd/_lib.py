@@ -140,6 +140,10 @@ SIGNATURES = {
     "tg_mesher_set_tuning": (C.c_int, [_p, _u32, _u32, _u32]),
     "tg_mesher_write_stl": (C.c_int, [_p, _p, C.c_char_p, _f, _u32]),
     "tg_mesher_write_ply": (C.c_int, [_p, _p, C.c_char_p, _f, _u32]),
+    "tg_mesher_grid_get": (C.c_int, [_p, C.POINTER(TgGrid)]),
+    "tg_mesher_export_sodapop": (C.c_int, [_p, _p, _fp, _f, C.c_int, _u32]),
+    "tg_mesher_copy_normals": (C.c_int, [_p, _fp]),
+    "tg_mesher_export_points": (C.c_int, [_p, _p, _fp, _fp, _fp, C.c_int, _u32]),
     "tg_batch_create": (C.c_int, [C.POINTER(_p)]),
     "tg_batch_destroy": (None, [_p]),
     "tg_batch_export": (C.c_int, [_p, C.POINTER(_p), C.POINTER(TgGrid), _u32, _u32]),

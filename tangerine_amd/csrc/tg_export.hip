@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <random>
 #include <vector>
 #include <cmath>
 #include <cstdio>
@@ -34,6 +35,9 @@ struct tg_mesher
 	hipStream_t stream = nullptr;
 	hipEvent_t ev[6] = {};
 	DevBuf pool, signs, active, word_count, word_base, quad_count, quad_base, vertices, triangles, scan_tmp, dense, cubes;
+	DevBuf normals, exchange;
+	DevBuf cloud_flags, cloud_rank;
+	bool have_normals = false;
 	DevBuf counters;
 	DevBuf brick_flags, brick_active, tile_mask, block_list, tile_list;
 	uint64_t nbricks = 0;         // bricks of the last culled export (0: none)
@@ -338,7 +342,8 @@ extern "C" void tg_mesher_destroy(tg_mesher* m)
 	if (m->stream) (void)hipStreamSynchronize(m->stream);
 	for (DevBuf* b : { &m->pool, &m->signs, &m->active, &m->word_count, &m->word_base, &m->quad_count, &m->quad_base,
 	                   &m->vertices, &m->triangles, &m->scan_tmp, &m->dense, &m->counters, &m->brick_flags,
-	                   &m->brick_active, &m->tile_mask, &m->block_list, &m->tile_list, &m->cubes })
+	                   &m->brick_active, &m->tile_mask, &m->block_list, &m->tile_list, &m->cubes, &m->normals,
+	                   &m->exchange })
 		b->release();
 	m->blocks.release();
 	for (hipEvent_t e : m->ev)
@@ -464,6 +469,7 @@ int run_export(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fla
 			return fail(TG_ERR_INVALID, "culled export: surrogate is not calibrated");
 	}
 	m->have_grid = false;
+	m->have_normals = false;
 	m->nbricks = 0;
 	rc = layout(m, g);
 	if (rc) return rc;
@@ -919,6 +925,233 @@ extern "C" int tg_mesher_write_ply(tg_mesher* m, const tg_program* p, const char
 		}
 	}
 	if (!ok) return fail(TG_ERR_INVALID, std::string("tg_mesher_write_ply: write failed: ") + path);
+	return TG_OK;
+}
+
+// ---- Sodapop meshing job (tangerine/sodapop.cpp:156-177, 337-401, 562-897) ---
+
+namespace {
+
+// The grid of NaiveSurfaceNetsScratch: the bounding cube of the model's
+// bounds (AABB::BoundingCube, aabb.cpp:122-139), floor(density) samples per
+// unit and at least 8 per axis, origin moved down two steps, three more cells.
+int sodapop_grid(const float b[6], float density, tg_grid* g)
+{
+	float mn[3], mx[3], ext[3];
+	for (int i = 0; i < 3; ++i)
+	{
+		mn[i] = b[i];
+		mx[i] = b[3 + i];
+		if (!std::isfinite(mn[i]) || !std::isfinite(mx[i]) || !(mx[i] > mn[i]))
+			return fail(TG_ERR_INVALID, "sodapop export: bounds must be finite and well formed");
+		ext[i] = mx[i] - mn[i];
+	}
+	const float longest = std::max(std::max(ext[0], ext[1]), ext[2]);
+	for (int i = 0; i < 3; ++i)
+	{
+		const float pad = (longest - ext[i]) * 0.5f;
+		mn[i] = mn[i] - pad;
+		mx[i] = mx[i] + pad;
+		ext[i] = mx[i] - mn[i];
+	}
+	const float d = std::floor(density);
+	float o[3], step[3];
+	uint64_t cells[3];
+	for (int i = 0; i < 3; ++i)
+	{
+		const float spu = std::max(ext[i] * d, 8.0f);
+		cells[i] = uint64_t(std::ceil(spu));
+		step[i] = ext[i] / float(cells[i]);
+		o[i] = mn[i] - step[i] * 2.0f;
+		cells[i] += 3;
+	}
+	*g = tg_grid{ o[0], o[1], o[2], step[0], step[1], step[2], cells[0], cells[1], cells[2] };
+	return TG_OK;
+}
+
+// ApplyVertexSequence (sodapop.cpp:337-394) with VertexSequence::Shuffle:
+// the first half of the vertices swaps with a shuffled (mt19937 seeded 0) list
+// of the second half; `exchange` maps every old index to its new one (an
+// involution).
+void vertex_sequence(uint64_t n, std::vector<uint32_t>& exchange)
+{
+	exchange.resize(n);
+	for (uint64_t i = 0; i < n; ++i) exchange[i] = uint32_t(i);
+	const size_t half = n / 2, mirror = (n % 2 == 0) ? half : half + 1;
+	std::vector<size_t> seq;
+	seq.reserve(mirror);
+	for (size_t k = 0; k < half; ++k) seq.push_back(mirror + k);
+	std::mt19937 gen;
+	gen.seed(0);
+	std::shuffle(seq.begin(), seq.end(), gen);
+	if (mirror != half) seq.push_back(half);
+	for (size_t t = 0; t < seq.size(); ++t)
+	{
+		exchange[t] = uint32_t(seq[t]);
+		exchange[seq[t]] = uint32_t(t);
+	}
+}
+
+__global__ void __launch_bounds__(256) k_permute_vertices(const float* src, float* dst, const uint32_t* exchange,
+	uint64_t n)
+{
+	const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const uint32_t from = exchange[i];  // an involution: new i holds old exchange[i]
+	dst[3 * i] = src[3 * size_t(from)];
+	dst[3 * i + 1] = src[3 * size_t(from) + 1];
+	dst[3 * i + 2] = src[3 * size_t(from) + 2];
+}
+
+__global__ void __launch_bounds__(256) k_remap_indices(uint32_t* tri, const uint32_t* exchange, uint64_t n)
+{
+	const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (i < n) tri[i] = exchange[tri[i]];
+}
+
+}  // namespace
+
+extern "C" int tg_mesher_export_sodapop(tg_mesher* m, const tg_program* p, const float bounds[6], float density,
+	int shuffle, uint32_t flags)
+{
+	if (!m || !p || !bounds) return fail(TG_ERR_INVALID, "tg_mesher_export_sodapop: null argument");
+	if (!(density >= 1.0f)) return fail(TG_ERR_INVALID, "tg_mesher_export_sodapop: density must be >= 1");
+	tg_grid g{};
+	int rc = sodapop_grid(bounds, density, &g);
+	if (rc) return rc;
+	m->have_normals = false;
+	rc = run_export(m, p, &g, flags, false, nullptr);
+	if (rc) return rc;
+	const uint64_t nv = m->nverts, ni = m->ntris * 3;
+	if (shuffle && nv > 1)
+	{
+		std::vector<uint32_t> ex;
+		vertex_sequence(nv, ex);
+		rc = m->exchange.ensure(nv * 4);
+		if (!rc) rc = m->normals.ensure(nv * 12);  // staging for the permuted positions
+		if (rc) return rc;
+		TG_CHECK_HIP(hipMemcpyAsync(m->exchange.ptr, ex.data(), nv * 4, hipMemcpyHostToDevice, m->stream), "upload");
+		hipLaunchKernelGGL(k_permute_vertices, dim3(grid_for(nv)), dim3(256), 0, m->stream,
+			m->vertices.as<const float>(), m->normals.as<float>(), m->exchange.as<const uint32_t>(), nv);
+		TG_CHECK_HIP(hipGetLastError(), "permute launch");
+		TG_CHECK_HIP(hipMemcpyAsync(m->vertices.ptr, m->normals.ptr, nv * 12, hipMemcpyDeviceToDevice, m->stream),
+			"permute copy");
+		if (ni)
+		{
+			hipLaunchKernelGGL(k_remap_indices, dim3(grid_for(ni)), dim3(256), 0, m->stream, m->triangles.as<uint32_t>(),
+				m->exchange.as<const uint32_t>(), ni);
+			TG_CHECK_HIP(hipGetLastError(), "remap launch");
+		}
+		TG_CHECK_HIP(hipStreamSynchronize(m->stream), "vertex sequence");
+	}
+	// normals: SDFOctree::Gradient at every (final) vertex (USE_GRADIENT_NORMALS)
+	std::vector<float> normals;
+	rc = gradients_at(m, p, m->vertices.as<const float>(), nv, flags, normals);
+	if (rc) return rc;
+	rc = m->normals.ensure(std::max<uint64_t>(nv, 1) * 12);
+	if (rc) return rc;
+	if (nv) TG_CHECK_HIP(hipMemcpy(m->normals.ptr, normals.data(), nv * 12, hipMemcpyHostToDevice), "normals");
+	m->have_normals = true;
+	return TG_OK;
+}
+
+extern "C" int tg_mesher_copy_normals(const tg_mesher* m, float* normals)
+{
+	if (!m || !normals) return fail(TG_ERR_INVALID, "tg_mesher_copy_normals: null argument");
+	if (!m->have_normals) return fail(TG_ERR_INVALID, "tg_mesher_copy_normals: no normals (run a sodapop export)");
+	if (m->nverts) TG_CHECK_HIP(hipMemcpy(normals, m->normals.ptr, m->nverts * 12, hipMemcpyDeviceToHost), "normals");
+	return TG_OK;
+}
+
+// ---- point-cloud export (tangerine/export.cpp:384-480) --------------------------
+
+extern "C" int tg_mesher_export_points(tg_mesher* m, const tg_program* p, const float lo[3], const float hi[3],
+	const float step[3], int refine_iterations, uint32_t flags)
+{
+	if (!m || !p || !lo || !hi || !step) return fail(TG_ERR_INVALID, "tg_mesher_export_points: null argument");
+	if (refine_iterations < 0) return fail(TG_ERR_INVALID, "tg_mesher_export_points: negative refine iterations");
+	CloudArgs a{};
+	uint64_t n[3];
+	float half[3];
+	for (int i = 0; i < 3; ++i)
+	{
+		if (!std::isfinite(lo[i]) || !std::isfinite(hi[i]) || !(step[i] > 0.0f) || !std::isfinite(step[i]))
+			return fail(TG_ERR_INVALID, "tg_mesher_export_points: bounds and step must be finite, step positive");
+		// ivec3(ceil((Stop - Start) / Step)); no cells on an axis with Stop <= Start
+		const float c = std::ceil((hi[i] - lo[i]) / step[i]);
+		if (!(c < 2147483647.0f)) return fail(TG_ERR_LIMIT, "tg_mesher_export_points: too many cells per axis");
+		n[i] = c > 0.0f ? uint64_t(c) : 0;
+		half[i] = step[i] / 2.0f;
+	}
+	const uint64_t cells = n[0] * n[1] * n[2];
+	if (cells >= (1ull << 31)) return fail(TG_ERR_LIMIT, "tg_mesher_export_points: more than 2^31-1 cells");
+	a.prog = static_cast<const Instr*>(p->device.ptr);
+	a.sx = lo[0]; a.sy = lo[1]; a.sz = lo[2];
+	a.dx = step[0]; a.dy = step[1]; a.dz = step[2];
+	a.hx = half[0]; a.hy = half[1]; a.hz = half[2];
+	a.ix = uint32_t(n[0]); a.iy = uint32_t(n[1]); a.iz = uint32_t(n[2]);
+	a.cells = cells;
+	a.diagonal = std::sqrt((half[0] * half[0] + half[1] * half[1]) + half[2] * half[2]);  // glm length
+	a.iterations = refine_iterations;
+	a.slots = std::max<uint32_t>(1, p->decoded.max_stack - 1);
+	m->have_grid = false;
+	m->have_normals = false;
+	m->nbricks = 0;
+	m->nverts = m->ntris = 0;
+	m->stats = tg_export_stats{};
+	m->stats.samples = cells;
+	const bool fast = (flags & TG_FLAG_FAST) != 0;
+	int rc = m->cloud_flags.ensure(std::max<uint64_t>(cells, 1) * 4);
+	if (!rc) rc = m->cloud_rank.ensure(std::max<uint64_t>(cells, 1) * 4);
+	size_t tmp = 0;
+	if (rc) return rc;
+	TG_CHECK_HIP(exclusive_scan_u32(nullptr, nullptr, cells, nullptr, tmp, m->stream), "scan sizing");
+	rc = m->scan_tmp.ensure(tmp);
+	if (rc) return rc;
+	a.flags = m->cloud_flags.as<uint32_t>();
+	a.rank = m->cloud_rank.as<const uint32_t>();
+	TG_CHECK_HIP(hipEventRecord(m->ev[0], m->stream), "event");
+	uint64_t np = 0;
+	if (cells)
+	{
+		hipError_t e = fast ? fast::launch_cloud(a, 0, m->stream) : exact::launch_cloud(a, 0, m->stream);
+		if (e != hipSuccess) return hip_fail(e, "point select launch");
+		tmp = m->scan_tmp.bytes;
+		TG_CHECK_HIP(exclusive_scan_u32(a.flags, m->cloud_rank.as<uint32_t>(), cells, m->scan_tmp.ptr, tmp, m->stream),
+			"point scan");
+		uint32_t tail[2] = { 0, 0 };
+		TG_CHECK_HIP(hipMemcpyAsync(&tail[0], a.rank + cells - 1, 4, hipMemcpyDeviceToHost, m->stream), "count copy");
+		TG_CHECK_HIP(hipMemcpyAsync(&tail[1], a.flags + cells - 1, 4, hipMemcpyDeviceToHost, m->stream), "count copy");
+		TG_CHECK_HIP(hipStreamSynchronize(m->stream), "point count");
+		np = uint64_t(tail[0]) + tail[1];
+	}
+	rc = m->vertices.ensure(std::max<uint64_t>(np, 1) * 12);
+	if (rc) return rc;
+	a.points = m->vertices.as<float>();
+	a.npoints = np;
+	if (np)
+	{
+		hipError_t e = fast ? fast::launch_cloud(a, 1, m->stream) : exact::launch_cloud(a, 1, m->stream);
+		if (e == hipSuccess && refine_iterations > 0)
+			e = fast ? fast::launch_cloud(a, 2, m->stream) : exact::launch_cloud(a, 2, m->stream);
+		if (e != hipSuccess) return hip_fail(e, "point refine launch");
+	}
+	TG_CHECK_HIP(hipEventRecord(m->ev[1], m->stream), "event");
+	TG_CHECK_HIP(hipEventSynchronize(m->ev[1]), "point cloud");
+	float ms = 0.0f;
+	TG_CHECK_HIP(hipEventElapsedTime(&ms, m->ev[0], m->ev[1]), "event time");
+	m->nverts = np;
+	m->stats.eval_ms = ms;
+	m->stats.total_ms = ms;
+	m->stats.exact_evals = cells + np * uint64_t(refine_iterations) * 5;
+	return TG_OK;
+}
+
+extern "C" int tg_mesher_grid_get(const tg_mesher* m, tg_grid* out)
+{
+	if (!m || !out) return fail(TG_ERR_INVALID, "tg_mesher_grid_get: null argument");
+	const MeshArgs& a = m->args;
+	*out = tg_grid{ a.gx, a.gy, a.gz, a.dx, a.dy, a.dz, a.sx, a.sy, a.sz };
 	return TG_OK;
 }
 

@@ -195,6 +195,25 @@ struct TraceArgs
 	uint32_t slots;
 };
 
+// Point-cloud export: cells of a regular point grid kept near the surface,
+// then refined along the gradient.
+struct CloudArgs
+{
+	const Instr* prog;
+	float sx, sy, sz;          // start (model minimum)
+	float dx, dy, dz;          // step
+	float hx, hy, hz;          // half step
+	uint32_t ix, iy, iz;       // cells per axis
+	uint64_t cells;
+	float diagonal;            // length of the half step
+	uint32_t* flags;           // per cell: kept
+	const uint32_t* rank;      // exclusive scan of flags
+	float* points;             // 3 per kept cell
+	uint64_t npoints;
+	int iterations;            // refinement steps
+	uint32_t slots;
+};
+
 // Central-difference or tetrahedral gradients at points.
 struct GradArgs
 {
@@ -211,6 +230,7 @@ struct GradArgs
 // fast = contracted FP32).  `p` is points per lane for the tile kernel (4 or 8).
 namespace exact {
 hipError_t launch_points(const PointArgs& a, hipStream_t s);
+hipError_t launch_cloud(const CloudArgs& a, int stage, hipStream_t s);  // 0 select, 1 points, 2 refine
 hipError_t launch_batch(const BatchArgs& b, int p, uint32_t waves, uint32_t workgroups, hipStream_t s);
 hipError_t launch_batch_reset(unsigned int* head, unsigned long long* work, uint32_t nwork, hipStream_t s);
 hipError_t launch_certify(const CertifyArgs& a, hipStream_t s);
@@ -222,6 +242,7 @@ hipError_t launch_gradient(const GradArgs& a, hipStream_t s);
 }
 namespace fast {
 hipError_t launch_points(const PointArgs& a, hipStream_t s);
+hipError_t launch_cloud(const CloudArgs& a, int stage, hipStream_t s);  // 0 select, 1 points, 2 refine
 hipError_t launch_batch(const BatchArgs& b, int p, uint32_t waves, uint32_t workgroups, hipStream_t s);
 hipError_t launch_batch_reset(unsigned int* head, unsigned long long* work, uint32_t nwork, hipStream_t s);
 hipError_t launch_certify(const CertifyArgs& a, hipStream_t s);

@@ -125,6 +125,43 @@ class Mesher:
         check(self.lib.tg_mesher_write_ply(self.h, program.handle, str(path).encode(), float(scale),
                                            _lib.TG_FLAG_FAST if fast else 0), "tg_mesher_write_ply")
 
+    def export_sodapop(self, program, bounds, density: float = 20.0, shuffle: bool = True,
+                       fast: bool = False) -> dict:
+        """The interactive mesher's job (MeshingJob::NaiveSurfaceNets): its grid
+        over the bounding cube of `bounds` (2x3), optional vertex shuffle,
+        gradient normals (read with normals())."""
+        b = (C.c_float * 6)(*[float(v) for v in np.asarray(bounds, dtype=np.float32).reshape(-1)])
+        check(self.lib.tg_mesher_export_sodapop(self.h, program.handle, b, float(density), 1 if shuffle else 0,
+                                                _lib.TG_FLAG_FAST if fast else 0), "tg_mesher_export_sodapop")
+        g = self.grid_spec()
+        self._cells = g[2]
+        return self.stats()
+
+    def export_points(self, program, lo, hi, step, refine_iterations: int = 0, fast: bool = False) -> dict:
+        """Point cloud (PointCloudExportThread): centres of the cells of the grid
+        lo + i step within the half-step diagonal of the surface, refined
+        `refine_iterations` times along the gradient; read with mesh() (no
+        triangles), written with write_ply()."""
+        def f3(v):
+            a = np.broadcast_to(np.asarray(v, dtype=np.float32), (3,))
+            return (C.c_float * 3)(*[float(x) for x in a])
+        self._cells = (0, 0, 0)  # no sample grid: grid() is refused by the library
+        check(self.lib.tg_mesher_export_points(self.h, program.handle, f3(lo), f3(hi), f3(step), int(refine_iterations),
+                                               _lib.TG_FLAG_FAST if fast else 0), "tg_mesher_export_points")
+        return self.stats()
+
+    def grid_spec(self):
+        """(origin, step, cells) of the last export."""
+        g = _lib.TgGrid()
+        check(self.lib.tg_mesher_grid_get(self.h, C.byref(g)), "tg_mesher_grid_get")
+        return (g.x, g.y, g.z), (g.dx, g.dy, g.dz), (int(g.sx), int(g.sy), int(g.sz))
+
+    def normals(self) -> np.ndarray:
+        nv, _ = self.counts()
+        out = np.zeros((nv, 3), dtype=np.float32)
+        check(self.lib.tg_mesher_copy_normals(self.h, out.ctypes.data_as(_lib._fp)), "tg_mesher_copy_normals")
+        return out
+
     def grid(self) -> np.ndarray:
         nx, ny, nz = (c + 1 for c in self._cells)
         out = np.zeros(nx * ny * nz, dtype=np.float32)

@@ -138,6 +138,19 @@ def main() -> int:
         ref.free(hh)
     ref.free(h)
 
+    # ---- point clouds (PointCloudExportThread steps over the whole tree) ----
+    for name, spec, lo, hi, step, iters in [("config1", scenes.config1_spec(), (-2.0, -1.5, -1.5), (2.0, 1.5, 1.5),
+                                             0.05, (0, 2)),
+                                            ("rand16_s5", scenes.random_csg_spec(5, 16), (-6.0,) * 3, (6.0,) * 3,
+                                             0.15, (0, 1))]:
+        hh = ref.build(spec)
+        clouds = {f"points_it{it}": ref.point_cloud(hh, lo, hi, step, it) for it in iters}
+        np.savez_compressed(OUT / f"cloud_{name}.npz", spec=spec_array(spec), lo=np.array(lo, dtype=np.float32),
+                            hi=np.array(hi, dtype=np.float32), step=np.float32(step),
+                            iterations=np.array(iters, dtype=np.int64), **clouds)
+        summary[f"cloud_{name}"] = {k: int(len(v)) for k, v in clouds.items()}
+        ref.free(hh)
+
     (OUT / "summary.json").write_text(json.dumps(summary, indent=1) + "\n")
     print(json.dumps(summary, indent=1))
     return 0

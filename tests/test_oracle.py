@@ -291,3 +291,80 @@ def test_mlp_oracle_planted_mistake_is_caught():
     r = mlp.bf16(v)
     assert list(r[:3]) == [1.0, 1.5, -3.25]
     assert r[3] == 1.0 and r[4] == np.float32(1 + 2 ** -7)
+
+
+def test_vertex_sequence_oracle(oracle):
+    """ApplyVertexSequence (sodapop.cpp:337-394): the generator is the
+    standard's mt19937 (its 10000th output is fixed by the C++ standard), the
+    map is an involution swapping the first half with a shuffle of the second,
+    and the middle vertex of an odd count stays."""
+    assert oracle.mt19937_10000th() == 4123659995
+    for n in (0, 1, 2, 7, 10, 1001):
+        ex = oracle.vertex_sequence(n).astype(np.int64)
+        assert np.array_equal(ex[ex], np.arange(n))
+        half = n // 2
+        mirror = half if n % 2 == 0 else half + 1
+        assert np.all(ex[:half] >= mirror)
+        if n % 2:
+            assert ex[half] == half
+    a = oracle.vertex_sequence(1001)
+    assert not np.array_equal(a[:500], np.arange(501, 1001))   # actually shuffled
+    assert np.array_equal(a, oracle.vertex_sequence(1001))      # deterministic (seed 0)
+
+
+@pytest.mark.parametrize("name", ["cloud_config1.npz", "cloud_rand16_s5.npz"])
+def test_point_cloud_matches_reference(oracle, name):
+    """PointCloudExportThread's points.  The selection is bit-identical; so is
+    the refinement where the tree walk and the interpreter agree (config 1:
+    translations only).  With rotations the reference takes its refinement
+    direction from the tree walk, whose ulp-level differences from the
+    interpreter the h = 1e-4 tetrahedral gradient amplifies ~1e4 times: there
+    the bar is 99% of points within 2e-3 of the half-step diagonal times 2
+    and every point within the diagonal."""
+    g = load_golden(name)
+    h = oracle.build(golden_spec(g))
+    words = oracle.compile(h)
+    step = float(g["step"])
+    diag = float(np.linalg.norm(np.full(3, step / 2, dtype=np.float32)))
+    for it in g["iterations"]:
+        ref = g[f"points_it{int(it)}"]
+        got = oracle.point_cloud(words, g["lo"], g["hi"], step, int(it))
+        assert got.shape == ref.shape and len(ref) > 1000
+        if it == 0 or name == "cloud_config1.npz":
+            assert np.array_equal(bits(got), bits(ref))
+        else:
+            d = np.linalg.norm(got - ref, axis=1)
+            assert np.quantile(d, 0.99) <= 4e-3 * diag and d.max() <= diag
+    oracle.free(h)
+
+
+def test_point_cloud_closed_forms(oracle):
+    """Sphere of radius 1: the kept centres are exactly the float32 centres
+    with ||c| - 1| < |half|, in raster order; two refinement steps give the
+    radial projection c / |c| clamped to the cell (on the sphere where the
+    projection lies inside the cell); a planted
+    off-by-half-cell centre fails the first check."""
+    h = oracle.build([("sphere", 1.0)])
+    words = oracle.compile(h)
+    lo, hi, step = np.float32([-1.3, -1.2, -1.25]), np.float32([1.3, 1.2, 1.25]), np.float32(0.1)
+    f = np.float32
+    n = np.ceil((hi - lo) / step).astype(np.int64)
+    half = f(step / f(2))
+    diag = f(np.sqrt((half * half + half * half) + half * half))
+    k, j, i = np.meshgrid(*[np.arange(c, dtype=np.int64) for c in n[::-1]], indexing="ij")
+    cen = np.stack([(i.astype(f) * step + lo[0]) + half, (j.astype(f) * step + lo[1]) + half,
+                    (k.astype(f) * step + lo[2]) + half], axis=-1).reshape(-1, 3).astype(f)
+    d = oracle.interp_words(words, cen)
+    expect = cen[np.abs(d) < diag]
+    got = oracle.point_cloud(words, lo, hi, step, 0)
+    assert np.array_equal(got, expect) and len(got) > 500
+    shifted = oracle.point_cloud(words, lo + half, hi + half, step, 0)
+    assert shifted.shape != expect.shape or not np.array_equal(shifted, expect)
+    ref = oracle.point_cloud(words, lo, hi, step, 2)
+    c = expect.astype(np.float64)
+    radial = np.clip(c / np.linalg.norm(c, axis=1)[:, None], c - half, c + half)
+    # float32 tetrahedral differences (h = 1e-4) tilt the direction by ~1e-3
+    assert np.abs(ref - radial).max() < 2e-3 * diag
+    on = np.all(np.abs(radial - c) < half * (1 - 1e-4), axis=1)
+    assert on.sum() > 100 and np.abs(np.linalg.norm(ref[on].astype(np.float64), axis=1) - 1.0).max() < 1e-6
+    oracle.free(h)
