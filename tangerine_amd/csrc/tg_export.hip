@@ -84,11 +84,10 @@ int layout(tg_mesher* m, const tg_grid* g)
 	a.nx = a.sx + 1; a.ny = a.sy + 1; a.nz = a.sz + 1;
 	a.tx = (a.nx + 7) / 8; a.ty = (a.ny + 7) / 8; a.tz = (a.nz + 7) / 8;
 	a.cbx = (a.sx + 7) / 8; a.cby = (a.sy + 7) / 8; a.cbz = (a.sz + 7) / 8;
-	a.row_bytes = ((a.tx + 7) / 8) * 8;
 	a.aw = (a.sx + 63) / 64;
 	const uint64_t tiles = uint64_t(a.tx) * a.ty * a.tz;
 	int rc = m->pool.ensure(tiles * kTileSamples * sizeof(float));
-	if (!rc) rc = m->signs.ensure(uint64_t(a.ny) * a.nz * a.row_bytes);
+	if (!rc) rc = m->signs.ensure(tiles * 64);
 	if (rc) return rc;
 	a.pool = m->pool.as<float>();
 	a.signs = m->signs.as<uint8_t>();
@@ -265,7 +264,6 @@ BlockEvalArgs block_args(tg_mesher* m, const tg_program* p, const tg_grid* g, co
 	ba.tile_slot = ma.tile_slot;
 	ba.out_pool = m->pool.as<float>();
 	ba.signs = m->signs.as<uint8_t>();
-	ba.row_bytes = ma.row_bytes;
 	ba.slots = std::max<uint32_t>(1, m->blocks.max_stack > 0 ? m->blocks.max_stack - 1 : 1);
 	ba.prune_tiles = 1;
 	ba.work = work;
@@ -485,7 +483,6 @@ int run_export(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fla
 	ga.tile_slot = nullptr;
 	ga.pool = m->pool.as<float>();
 	ga.signs = m->signs.as<uint8_t>();
-	ga.row_bytes = ma.row_bytes;
 	ga.slots = std::max<uint32_t>(1, p->decoded.max_stack - 1);
 
 	TG_CHECK_HIP(hipEventRecord(m->ev[0], m->stream), "event");

@@ -34,8 +34,7 @@ struct GridArgs
 	const uint32_t* tile_list;  // tile ids to evaluate, or null for all
 	const uint32_t* tile_slot;  // pool slot per tile id, or null for identity
 	float* pool;             // 512 floats per slot, [z][y][x] inside a tile
-	uint8_t* signs;          // sign bit rows: byte (k*ny + j)*row_bytes + tile_x
-	uint32_t row_bytes;
+	uint8_t* signs;          // sign bits: 64 bytes per tile id, byte (z & 7) * 8 + (y & 7), x bits LSB first
 	uint32_t slots;
 };
 
@@ -108,8 +107,7 @@ struct BlockEvalArgs
 	uint32_t bx, by, bz;
 	const uint32_t* tile_slot;
 	float* out_pool;
-	uint8_t* signs;
-	uint32_t row_bytes;
+	uint8_t* signs;              // 64 bytes per tile id (see GridArgs)
 	uint32_t slots;              // LDS spill slots (max block stack - 1)
 	int prune_tiles;             // 0: evaluate block programs unpruned per tile
 	uint32_t min_prune_len;      // blocks with shorter programs skip the per-tile pruning

@@ -17,8 +17,7 @@ struct MeshArgs
 	// inputs
 	const float* pool;
 	const uint32_t* tile_slot;    // null: identity
-	const uint8_t* signs;
-	uint32_t row_bytes;
+	const uint8_t* signs;         // 64 bytes per tile id (tg_mesh.hip)
 	const uint8_t* brick_flags;   // null: every cube brick may hold active cubes
 	// intermediates
 	uint32_t aw;                  // 64-bit words per cube row

@@ -21,8 +21,9 @@
 // Data layout (built by the evaluation kernels):
 //   pool  : 8^3-sample tiles, 512 floats each, [z][y][x] inside a tile; the
 //           slot of tile t is tile_slot[t] (identity when null).
-//   signs : one bit per sample (1 = s >= 0); row (j, k) of the sample grid is
-//           row_bytes bytes at (k * ny + j) * row_bytes, x bit-packed LSB first.
+//   signs : one bit per sample (1 = s >= 0), 64 bytes per tile id (not slot):
+//           byte (k & 7) * 8 + (j & 7) of tile (i >> 3, j >> 3, k >> 3) holds
+//           x = 8 (i >> 3) .. + 7, LSB first (one wave's ballot per z-slice).
 //   active: one bit per cube; cube row (j, k) is `aw` 64-bit words at
 //           (k * sy + j) * aw.
 
@@ -44,12 +45,6 @@ __device__ __forceinline__ float pool_at(const MeshArgs& a, uint32_t i, uint32_t
 	return a.pool[size_t(slot) * kTileSamples + ((k & 7) * 64 + (j & 7) * 8 + (i & 7))];
 }
 
-__device__ __forceinline__ uint64_t sign_word(const MeshArgs& a, uint32_t j, uint32_t k, uint32_t w)
-{
-	const uint64_t* row = reinterpret_cast<const uint64_t*>(a.signs + (size_t(k) * a.ny + j) * a.row_bytes);
-	return (w * 8u < a.row_bytes) ? row[w] : 0ull;
-}
-
 __device__ __forceinline__ bool cube_active(const MeshArgs& a, uint32_t i, uint32_t j, uint32_t k)
 {
 	const uint64_t word = a.active[(size_t(k) * a.sy + j) * a.aw + (i >> 6)];
@@ -63,47 +58,89 @@ __device__ __forceinline__ uint32_t vertex_id(const MeshArgs& a, uint32_t i, uin
 	return a.word_base[wi] + uint32_t(__popcll(below));
 }
 
-// Thread per (cube row, word): active bits and their count.
-__global__ void __launch_bounds__(256) k_classify(MeshArgs a)
+// Workgroup per (segment of 8 active words = 64 cube columns, 8 cube rows in
+// y, 8 in z).  Task (tile lt, slice kk) reads the 8-byte z-slices it needs
+// from the tile-major sign array (8 consecutive tasks read one tile's 64
+// bytes) and forms the active bits of its 8 x 8 cubes byte-parallel in a
+// 64-bit word; the bytes are transposed through LDS into raster-order active
+// words (a cube is active when its 8 corner signs differ).
+constexpr uint32_t kSegWords = 8;  // active words per segment (64 sample tiles)
+
+__device__ __forceinline__ uint64_t sign_slice(const MeshArgs& a, uint32_t tix, uint32_t tiy, uint32_t tiz, uint32_t z)
 {
-	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-	const uint64_t rows = uint64_t(a.sy) * a.sz;
-	if (t >= rows * a.aw) return;
-	const uint32_t w = uint32_t(t % a.aw);
-	const uint64_t row = t / a.aw;
-	const uint32_t j = uint32_t(row % a.sy), k = uint32_t(row / a.sy);
-	const uint64_t s0 = sign_word(a, j, k, w), s1 = sign_word(a, j + 1, k, w);
-	const uint64_t s2 = sign_word(a, j, k + 1, w), s3 = sign_word(a, j + 1, k + 1, w);
-	const uint64_t n0 = sign_word(a, j, k, w + 1), n1 = sign_word(a, j + 1, k, w + 1);
-	const uint64_t n2 = sign_word(a, j, k + 1, w + 1), n3 = sign_word(a, j + 1, k + 1, w + 1);
-	const uint64_t all0 = s0 & s1 & s2 & s3, any0 = s0 | s1 | s2 | s3;
-	const uint64_t all1 = n0 & n1 & n2 & n3, any1 = n0 | n1 | n2 | n3;
-	// Corner x = i and x = i + 1: shift the next-sample bits down by one.
-	const uint64_t all_hi = (all0 >> 1) | (all1 << 63);
-	const uint64_t any_hi = (any0 >> 1) | (any1 << 63);
-	uint64_t act = ~(all0 & all_hi) & (any0 | any_hi);
-	const uint64_t first = uint64_t(w) * 64;
-	if (first + 64 > a.sx)
+	if (tix >= a.tx || tiy >= a.ty || tiz >= a.tz) return 0ull;
+	const size_t tile = tix + size_t(a.tx) * (tiy + size_t(a.ty) * tiz);
+	return reinterpret_cast<const uint64_t*>(a.signs)[tile * 8 + z];
+}
+
+// Active bits of the 8 x 8 cubes of tile column tix at cube slice kk: byte jj
+// holds cubes x = 8 tix .. 8 tix + 7 of row (8 cy + jj, 8 cz + kk).
+__device__ __forceinline__ void corner_masks(const MeshArgs& a, uint32_t tix, uint32_t cy, uint32_t cz, uint32_t kk,
+	uint64_t& all, uint64_t& any)
+{
+	const uint64_t s = sign_slice(a, tix, cy, cz, kk);
+	const uint64_t sz = kk < 7 ? sign_slice(a, tix, cy, cz, kk + 1) : sign_slice(a, tix, cy, cz + 1, 0);
+	const uint64_t y0 = sign_slice(a, tix, cy + 1, cz, kk) & 0xffull;
+	const uint64_t y0z = (kk < 7 ? sign_slice(a, tix, cy + 1, cz, kk + 1) : sign_slice(a, tix, cy + 1, cz + 1, 0)) & 0xffull;
+	const uint64_t s_up = (s >> 8) | (y0 << 56), sz_up = (sz >> 8) | (y0z << 56);  // rows jj + 1
+	all = s & s_up & sz & sz_up;
+	any = s | s_up | sz | sz_up;
+}
+
+__global__ void __launch_bounds__(256) k_classify(MeshArgs a, uint32_t nseg)
+{
+	constexpr uint32_t kTiles = kSegWords * 8 + 1;  // + the next tile, for the x + 1 corner
+	__shared__ uint64_t lds[8 * 8 * kSegWords];      // [kk][jj][8 tiles per word] bytes
+	__shared__ uint64_t all_s[kTiles * 8], any_s[kTiles * 8];
+	const uint32_t seg = blockIdx.x % nseg;
+	const uint32_t cy = (blockIdx.x / nseg) % a.cby, cz = blockIdx.x / (nseg * a.cby);
+	const uint32_t t0 = seg * kSegWords * 8;
+	for (uint32_t q = threadIdx.x; q < kTiles * 8; q += blockDim.x)
+		corner_masks(a, t0 + (q >> 3), cy, cz, q & 7, all_s[q], any_s[q]);
+	__syncthreads();
+	uint8_t* bytes = reinterpret_cast<uint8_t*>(lds);
+	for (uint32_t q = threadIdx.x; q < 512; q += blockDim.x)
 	{
-		const uint32_t keep = (a.sx > first) ? uint32_t(a.sx - first) : 0u;
-		act &= (keep >= 64) ? ~0ull : ((1ull << keep) - 1ull);
-	}
-	if (a.brick_flags)
-	{
-		// Only cube bricks (8^3 cubes) flagged for exact evaluation may hold
-		// active cubes; the flags hold one byte per brick.
-		const uint32_t bj = j >> 3, bk = k >> 3;
-		uint64_t m = 0;
+		const uint32_t kk = q & 7, lt = q >> 3;
+		const uint64_t all0 = all_s[q], any0 = any_s[q], all1 = all_s[q + 8], any1 = any_s[q + 8];
+		// corner x = i + 1: shift each byte down by one, bit 7 from the next tile's bit 0
+		const uint64_t lo7 = 0x7f7f7f7f7f7f7f7full, b0 = 0x0101010101010101ull;
+		const uint64_t all_hi = ((all0 >> 1) & lo7) | ((all1 & b0) << 7);
+		const uint64_t any_hi = ((any0 >> 1) & lo7) | ((any1 & b0) << 7);
+		const uint64_t act = ~(all0 & all_hi) & (any0 | any_hi);
 #pragma unroll
-		for (int b = 0; b < 8; ++b)
-		{
-			const uint32_t bi = w * 8 + b;
-			if (bi < a.cbx && a.brick_flags[bi + a.cbx * (bj + a.cby * bk)]) m |= 0xffull << (8 * b);
-		}
-		act &= m;
+		for (uint32_t jj = 0; jj < 8; ++jj) bytes[(kk * 8 + jj) * 64 + lt] = uint8_t(act >> (8 * jj));
 	}
-	a.active[t] = act;
-	a.word_count[t] = uint32_t(__popcll(act));
+	__syncthreads();
+	for (uint32_t o = threadIdx.x; o < 512; o += blockDim.x)
+	{
+		const uint32_t ww = o & 7, jj = (o >> 3) & 7, kk = o >> 6;
+		const uint32_t w = seg * kSegWords + ww, j = cy * 8 + jj, k = cz * 8 + kk;
+		if (w >= a.aw || j >= a.sy || k >= a.sz) continue;
+		uint64_t act = lds[(kk * 8 + jj) * 8 + ww];
+		const uint64_t first = uint64_t(w) * 64;
+		if (first + 64 > a.sx)
+		{
+			const uint32_t keep = (a.sx > first) ? uint32_t(a.sx - first) : 0u;
+			act &= (keep >= 64) ? ~0ull : ((1ull << keep) - 1ull);
+		}
+		if (a.brick_flags)
+		{
+			// Only cube bricks (8^3 cubes) flagged for exact evaluation may hold
+			// active cubes; the flags hold one byte per brick.
+			uint64_t m = 0;
+#pragma unroll
+			for (int b = 0; b < 8; ++b)
+			{
+				const uint32_t bi = w * 8 + b;
+				if (bi < a.cbx && a.brick_flags[bi + a.cbx * (cy + a.cby * cz)]) m |= 0xffull << (8 * b);
+			}
+			act &= m;
+		}
+		const size_t t = (size_t(k) * a.sy + j) * a.aw + w;
+		a.active[t] = act;
+		a.word_count[t] = uint32_t(__popcll(act));
+	}
 }
 
 // Vertex of active cube (i, j, k): the mean of the edge crossings, mapped to
@@ -276,18 +313,20 @@ __global__ void __launch_bounds__(256) k_dense_to_pool(MeshArgs a, const float* 
 
 __global__ void __launch_bounds__(256) k_dense_signs(MeshArgs a, const float* dense, uint8_t* signs)
 {
-	// Thread per sign byte.
+	// Thread per sign byte: tile t / 64, row (j & 7) = t & 7, (k & 7) = (t / 8) & 7.
 	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-	const uint64_t rows = uint64_t(a.ny) * a.nz;
-	if (t >= rows * a.row_bytes) return;
-	const uint32_t b = uint32_t(t % a.row_bytes);
-	const uint64_t row = t / a.row_bytes;
+	if (t >= uint64_t(a.tx) * a.ty * a.tz * 64) return;
+	const uint64_t tile = t / 64;
+	const uint32_t tix = uint32_t(tile % a.tx), tiy = uint32_t((tile / a.tx) % a.ty);
+	const uint32_t tiz = uint32_t(tile / (uint64_t(a.tx) * a.ty));
+	const uint32_t j = tiy * 8 + uint32_t(t & 7), k = tiz * 8 + uint32_t((t >> 3) & 7);
 	uint8_t byte = 0;
-	for (int x = 0; x < 8; ++x)
-	{
-		const uint32_t i = b * 8 + x;
-		if (i < a.nx && dense[row * a.nx + i] >= 0.0f) byte |= uint8_t(1u << x);
-	}
+	if (j < a.ny && k < a.nz)
+		for (int x = 0; x < 8; ++x)
+		{
+			const uint32_t i = tix * 8 + x;
+			if (i < a.nx && dense[(size_t(k) * a.ny + j) * a.nx + i] >= 0.0f) byte |= uint8_t(1u << x);
+		}
 	signs[t] = byte;
 }
 
@@ -297,8 +336,11 @@ inline unsigned blocks_for(uint64_t n, unsigned threads) { return unsigned((n + 
 
 hipError_t mesh_classify(const MeshArgs& a, hipStream_t s)
 {
-	const uint64_t n = uint64_t(a.sy) * a.sz * a.aw;
-	hipLaunchKernelGGL(k_classify, dim3(blocks_for(n, 256)), dim3(256), 0, s, a);
+	const uint32_t nseg = (a.aw + kSegWords - 1) / kSegWords;
+	const uint64_t blocks = uint64_t(nseg) * a.cby * a.cbz;
+	if (blocks == 0) return hipSuccess;
+	if (blocks >= (1ull << 31)) return hipErrorInvalidValue;
+	hipLaunchKernelGGL(k_classify, dim3(unsigned(blocks)), dim3(256), 0, s, a, nseg);
 	return hipGetLastError();
 }
 
@@ -341,7 +383,7 @@ hipError_t mesh_dense_to_pool(const MeshArgs& a, const float* dense, hipStream_t
 {
 	const uint64_t n = uint64_t(a.nx) * a.ny * a.nz;
 	hipLaunchKernelGGL(k_dense_to_pool, dim3(blocks_for(n, 256)), dim3(256), 0, s, a, dense, const_cast<float*>(a.pool));
-	const uint64_t m = uint64_t(a.ny) * a.nz * a.row_bytes;
+	const uint64_t m = uint64_t(a.tx) * a.ty * a.tz * 64;
 	hipLaunchKernelGGL(k_dense_signs, dim3(blocks_for(m, 256)), dim3(256), 0, s, a, dense, const_cast<uint8_t*>(a.signs));
 	return hipGetLastError();
 }
