@@ -347,7 +347,9 @@ def main(argv=None) -> int:
             "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
             "frac_of_nonpacked_peak": achieved / PEAK_FP32_NONPACKED_TFLOPS,
             "kernel": "k_eval_blocks" if prune else "k_eval_tiles", "flops_per_launch": flops, "launch_ms": launch_ms,
-            "launch": {"points_per_lane": st["points_per_lane"], "waves": st["waves"]},
+            "launch": {"points_per_lane": st["points_per_lane"], "waves": st["waves"],
+                       "lds_bytes": st["block_lds_bytes"], "block_max_len": st["block_max_len"],
+                       "block_max_binary": st["block_max_binary"]},
         },
     }
     if not args.no_cpu:

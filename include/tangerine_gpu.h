@@ -122,6 +122,9 @@ typedef struct tg_export_stats
 	double block_ms;           /* the pruned block-evaluation kernel alone (HIP events; 0 if it did not run) */
 	uint32_t points_per_lane;  /* its launch: points per lane and waves per workgroup */
 	uint32_t waves;
+	uint32_t block_max_len;    /* longest pruned block program (entries) and its binary entries */
+	uint32_t block_max_binary;
+	uint32_t block_lds_bytes;  /* LDS per workgroup of the block kernel */
 } tg_export_stats;
 
 typedef struct tg_surrogate_stats

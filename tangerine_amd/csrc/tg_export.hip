@@ -51,6 +51,7 @@ struct tg_mesher
 	uint32_t min_prune_len = 0;   // block kernel: shorter block programs skip per-tile pruning
 	int last_points_per_lane = 0;
 	int last_waves = 0;
+	uint32_t last_max_len = 0, last_max_binary = 0, last_lds = 0;
 	float last_block_ms = 0.0f;   // the last k_eval_blocks launch (HIP events)
 };
 
@@ -264,7 +265,8 @@ BlockEvalArgs block_args(tg_mesher* m, const tg_program* p, const tg_grid* g, co
 	ba.tile_slot = ma.tile_slot;
 	ba.out_pool = m->pool.as<float>();
 	ba.signs = m->signs.as<uint8_t>();
-	ba.slots = std::max<uint32_t>(1, m->blocks.max_stack > 0 ? m->blocks.max_stack - 1 : 1);
+	ba.max_depth = std::max<uint32_t>(1, m->blocks.max_stack);
+	ba.slots = block_spill_slots(ba.max_depth, m->points_per_lane);
 	ba.prune_tiles = 1;
 	ba.work = work;
 	ba.whole = p->entries.as<const Entry>();
@@ -289,8 +291,12 @@ int eval_blocks(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fl
 	BlockEvalArgs ba = block_args(m, p, g, tile_mask, block_list, n_listed, work);
 	int pts = m->points_per_lane;
 	// Fit the LDS: fewer points per lane first, then fewer waves.
-	while (pts > 2 && block_lds_bytes(ba.slots, pts, ba.waves, ba.max_binary, ba.max_len) > kMaxLds) pts >>= 1;
-	while (ba.waves > 1 && block_lds_bytes(ba.slots, pts, ba.waves, ba.max_binary, ba.max_len) > kMaxLds) ba.waves >>= 1;
+	auto lds_for = [&](int p, int w) {
+		return block_lds_bytes(block_spill_slots(ba.max_depth, p), p, w, ba.max_binary, ba.max_len);
+	};
+	while (pts > 2 && lds_for(pts, ba.waves) > kMaxLds) pts >>= 1;
+	while (ba.waves > 1 && lds_for(pts, ba.waves) > kMaxLds) ba.waves >>= 1;
+	ba.slots = block_spill_slots(ba.max_depth, pts);
 	if (block_lds_bytes(ba.slots, pts, ba.waves, ba.max_binary, ba.max_len) > kMaxLds)
 	{
 		// Block programs too long for per-tile lists in LDS: interpret them whole.
@@ -299,6 +305,9 @@ int eval_blocks(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fl
 	}
 	m->last_points_per_lane = pts;
 	m->last_waves = ba.waves;
+	m->last_max_len = ba.max_len;
+	m->last_max_binary = ba.max_binary;
+	m->last_lds = uint32_t(block_lds_bytes(ba.slots, pts, ba.waves, ba.max_binary, ba.max_len));
 	TG_CHECK_HIP(hipEventRecord(m->ev[4], m->stream), "event");
 	hipError_t e = (flags & TG_FLAG_FAST) ? fast::launch_eval_blocks(ba, pts, m->stream)
 	                                      : exact::launch_eval_blocks(ba, pts, m->stream);
@@ -545,6 +554,9 @@ int run_export(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fla
 	s.block_ms = blocks ? m->last_block_ms : 0.0;
 	s.points_per_lane = blocks ? uint32_t(m->last_points_per_lane) : 0u;
 	s.waves = blocks ? uint32_t(m->last_waves) : 0u;
+	s.block_max_len = blocks ? m->last_max_len : 0u;
+	s.block_max_binary = blocks ? m->last_max_binary : 0u;
+	s.block_lds_bytes = blocks ? m->last_lds : 0u;
 	if (culled)
 	{
 		s.bricks_total = m->nbricks;
@@ -1198,8 +1210,12 @@ int capture_batch(tg_batch* b)
 	drop_graph(b);
 	BatchArgs& ba = b->args;
 	int pts = b->points_per_lane, waves = b->waves;
-	while (pts > 2 && block_lds_bytes(ba.slots, pts, waves, ba.max_binary, ba.max_len) > kMaxLds) pts >>= 1;
-	while (waves > 1 && block_lds_bytes(ba.slots, pts, waves, ba.max_binary, ba.max_len) > kMaxLds) waves >>= 1;
+	auto lds_for = [&](int p, int w) {
+		return block_lds_bytes(block_spill_slots(ba.max_depth, p), p, w, ba.max_binary, ba.max_len);
+	};
+	while (pts > 2 && lds_for(pts, waves) > kMaxLds) pts >>= 1;
+	while (waves > 1 && lds_for(pts, waves) > kMaxLds) waves >>= 1;
+	ba.slots = block_spill_slots(ba.max_depth, pts);
 	if (block_lds_bytes(ba.slots, pts, waves, ba.max_binary, ba.max_len) > kMaxLds)
 		return fail(TG_ERR_LIMIT, "batch: block programs too long for the LDS");
 	const uint32_t wgs = uint32_t(std::max(1, b->cus * b->workgroups_per_cu));
@@ -1336,13 +1352,13 @@ extern "C" int tg_batch_export(tg_batch* b, const tg_program* const* programs, c
 		if (!m->blocks.valid) return fail(TG_ERR_LIMIT, "tg_batch_export: a tree could not be pruned");
 		args[i] = block_args(m, programs[i], &grids[i], nullptr, nullptr, 0, b->work.as<unsigned long long>() + 2 * i);
 		weight[i] = double(m->blocks.entries) + double(m->blocks.ncells);
-		ba.slots = std::max(ba.slots, args[i].slots);
+		ba.max_depth = std::max(ba.max_depth, args[i].max_depth);
 		ba.max_binary = std::max(ba.max_binary, args[i].max_binary);
 		ba.max_len = std::max(ba.max_len, args[i].max_len);
 		samples += uint64_t(m->args.nx) * m->args.ny * m->args.nz;
 	}
 	// Trees whose tables would not fit the LDS beside the others run unpruned tiles.
-	while (block_lds_bytes(ba.slots, 2, 1, ba.max_binary, ba.max_len) > kMaxLds)
+	while (block_lds_bytes(block_spill_slots(ba.max_depth, 2), 2, 1, ba.max_binary, ba.max_len) > kMaxLds)
 	{
 		uint32_t worst = 0;
 		for (uint32_t i = 1; i < count; ++i)

@@ -108,7 +108,8 @@ struct BlockEvalArgs
 	const uint32_t* tile_slot;
 	float* out_pool;
 	uint8_t* signs;              // 64 bytes per tile id (see GridArgs)
-	uint32_t slots;              // LDS spill slots (max block stack - 1)
+	uint32_t slots;              // LDS spill slots per wave, each P floats per lane (block_spill_slots)
+	uint32_t max_depth;          // deepest block program's stack (host: sizes `slots`)
 	int prune_tiles;             // 0: evaluate block programs unpruned per tile
 	uint32_t min_prune_len;      // blocks with shorter programs skip the per-tile pruning
 	unsigned long long* work;    // [0] samples interpreted, [1] flops units executed
@@ -130,6 +131,7 @@ struct BatchArgs
 	uint32_t total;              // queue items
 	unsigned int* head;          // queue head (reset to 0 before each launch)
 	uint32_t slots, max_binary, max_len;  // LDS layout: the largest over the trees
+	uint32_t max_depth;                   // deepest block program over the trees (host: sizes `slots`)
 };
 
 // Exact brick certificate: wave per block (4^3 cube bricks of 8^3 cubes,
@@ -254,6 +256,20 @@ hipError_t launch_gradient(const GradArgs& a, hipStream_t s);
 // LDS layout of the block kernel (bytes), shared by launcher and kernel:
 // decision masks (2 x max_binary) and liveness (max_len) of the block program,
 // then per wave a tile program (max_len words) and the spill region.
+// LDS spill slots of the block kernel, each P floats per lane.  A tile whose
+// program needs more than `slots` slots is interpreted one point per lane,
+// seeing the region as slots * P one-float slots, which must hold the deepest
+// block program (the wave-0 decision pass runs one point per lane too).
+// Beyond that minimum the region aims at about 3 KiB per wave: enough for
+// the short programs of most tiles, small enough for high occupancy.
+__host__ __device__ inline uint32_t block_spill_slots(uint32_t max_depth, int p)
+{
+	const uint32_t need = max_depth > 1 ? max_depth - 1 : 1;  // one-float slots
+	const uint32_t floor_slots = (need + uint32_t(p) - 1) / uint32_t(p);
+	const uint32_t budget = 3072u / (uint32_t(p) * 64u * 4u);
+	const uint32_t want = budget < need ? budget : need;
+	return floor_slots > want ? floor_slots : (want > 0 ? want : 1u);
+}
 __host__ __device__ inline size_t block_table_bytes(uint32_t max_binary, uint32_t max_len)
 {
 	return size_t(max_binary) * 16 + ((size_t(max_len) * 8 + 15) & ~size_t(15));
