@@ -213,6 +213,29 @@ int tg_gradient_points(const tg_program* program, const float* xyz, uint64_t cou
 int tg_raymarch(const tg_program* program, const float* starts, const float* dirs, uint64_t count,
                 int max_steps, float epsilon, float far_distance, uint32_t flags,
                 uint8_t* hit, float* travel, float* position);
+/* ---- Lua queries (tangerine/lua_sdf.cpp:386-559, sdf_model.cpp:150-264) --- */
+/* pivot_towards (LuaPivotTowards, lua_sdf.cpp:452-559) for `count` queries:
+ * pivots, headings, downs: 3 floats each; the scalar arguments as in Lua
+ * (max_angle in degrees).  Per query `status`: 0 done (tail and heading
+ * written), 2 heading antiparallel to down (the reference then draws random
+ * pivot axes from the Lua environment's generator: the caller does that),
+ * 3 the loop ran `max_iterations` turns (the reference would keep turning).
+ * Distances come from the program (interpreter), as eval does. */
+int tg_pivot_towards(const tg_program* program, uint64_t count, const float* pivots, const float* headings,
+                     const float* downs, float max_dist, float margin, float max_angle_degrees, int max_iterations,
+                     float* tails, float* headings_out, int32_t* status, uint32_t flags);
+/* Mouse picking (DeliverMouseButton with SDFModel::RayMarch, sdf_model.cpp:
+ * 150-206, 259-264) for `ray_count` rays against `model_count` models: each
+ * ray moved into every model's frame (transforms: 8 floats per model,
+ * rotation quaternion x, y, z, w, translation, scale; Transform::ApplyInv),
+ * marched 1000 steps with epsilon 0.001; the smallest local travel wins, the
+ * earlier model on ties.  model_out: index or -1; travel_out: INFINITY on a
+ * miss; position_out (3 floats per ray, may be NULL): the hit in the model's
+ * local frame, as the reference reports it. */
+int tg_pick(const tg_program* const* programs, const float* transforms, uint32_t model_count, const float* starts,
+            const float* dirs, uint64_t ray_count, uint32_t flags, int32_t* model_out, float* travel_out,
+            float* position_out);
+
 /* Camera looking from eye at target; up is replaced when parallel to the view. */
 int tg_camera_look_at(tg_camera* out, const float eye[3], const float target[3], const float up[3],
                       float fov_y_degrees, uint32_t width, uint32_t height);

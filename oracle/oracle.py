@@ -244,6 +244,36 @@ class Oracle(_Cpu):
         L.tgo_point_cloud(w.ctypes.data_as(_u32p), w.size, *[_fptr(v) for v in f3], int(iterations), _fptr(out), n)
         return out
 
+    def pivot_towards(self, words, max_dist, margin, max_angle_deg, pivot, heading, down, max_iterations=1 << 16):
+        """LuaPivotTowards over the interpreter: (status, tail, heading)."""
+        w = np.ascontiguousarray(words, dtype=np.uint32)
+        v = [np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(3)) for a in (pivot, heading, down)]
+        tail, head = np.zeros(3, np.float32), np.zeros(3, np.float32)
+        L = self.lib
+        L.tgo_pivot_towards_words.restype = C.c_int
+        L.tgo_pivot_towards_words.argtypes = [_u32p, _u64, _f, _f, _f, _fp, _fp, _fp, C.c_int, _fp, _fp]
+        st = L.tgo_pivot_towards_words(w.ctypes.data_as(_u32p), w.size, float(max_dist), float(margin),
+                                       float(max_angle_deg), *[_fptr(a) for a in v], int(max_iterations),
+                                       _fptr(tail), _fptr(head))
+        return st, tail, head
+
+    def pick(self, programs, transforms, start, direction):
+        """Nearest hit over models (DeliverMouseButton): (model, travel, local position)."""
+        words = [np.ascontiguousarray(p, dtype=np.uint32) for p in programs]
+        offs = np.zeros(len(words) + 1, dtype=np.uint64)
+        offs[1:] = np.cumsum([w.size for w in words])
+        allw = np.ascontiguousarray(np.concatenate(words) if words else np.zeros(0, np.uint32))
+        xf = np.ascontiguousarray(np.asarray(transforms, dtype=np.float32).reshape(-1, 8))
+        s = np.ascontiguousarray(np.asarray(start, dtype=np.float32).reshape(3))
+        d = np.ascontiguousarray(np.asarray(direction, dtype=np.float32).reshape(3))
+        tr, pos = np.zeros(1, np.float32), np.zeros(3, np.float32)
+        L = self.lib
+        L.tgo_pick_words.restype = C.c_int
+        L.tgo_pick_words.argtypes = [_u32p, _u64p, _fp, C.c_uint32, _fp, _fp, _fp, _fp]
+        m = L.tgo_pick_words(allw.ctypes.data_as(_u32p), offs.ctypes.data_as(_u64p), _fptr(xf), len(words), _fptr(s),
+                             _fptr(d), _fptr(tr), _fptr(pos))
+        return m, float(tr[0]), pos
+
     def mt19937_10000th(self) -> int:
         self.lib.tgo_mt19937_10000th.restype = C.c_uint32
         return int(self.lib.tgo_mt19937_10000th())
@@ -281,6 +311,28 @@ class Reference(_Cpu):
         L.tgref_mesh_evals.argtypes = [_p]
         L.tgref_threads_default.restype = C.c_int
         L.tgref_hardware_concurrency.restype = C.c_int
+
+    def pivot_towards(self, h, max_dist, margin, max_angle_deg, pivot, heading, down):
+        v = [np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(3)) for a in (pivot, heading, down)]
+        tail, head = np.zeros(3, np.float32), np.zeros(3, np.float32)
+        L = self.lib
+        L.tgref_pivot_towards.restype = C.c_int
+        L.tgref_pivot_towards.argtypes = [_p, _f, _f, _f, _fp, _fp, _fp, _fp, _fp]
+        st = L.tgref_pivot_towards(h, float(max_dist), float(margin), float(max_angle_deg), *[_fptr(a) for a in v],
+                                   _fptr(tail), _fptr(head))
+        return st, tail, head
+
+    def pick(self, handles, transforms, start, direction):
+        hs = (C.c_void_p * len(handles))(*handles)
+        xf = np.ascontiguousarray(np.asarray(transforms, dtype=np.float32).reshape(-1, 8))
+        s = np.ascontiguousarray(np.asarray(start, dtype=np.float32).reshape(3))
+        d = np.ascontiguousarray(np.asarray(direction, dtype=np.float32).reshape(3))
+        tr, pos = np.zeros(1, np.float32), np.zeros(3, np.float32)
+        L = self.lib
+        L.tgref_pick.restype = C.c_int
+        L.tgref_pick.argtypes = [C.c_void_p, _fp, C.c_uint32, _fp, _fp, _fp, _fp]
+        m = L.tgref_pick(C.cast(hs, C.c_void_p), _fptr(xf), len(handles), _fptr(s), _fptr(d), _fptr(tr), _fptr(pos))
+        return m, float(tr[0]), pos
 
     def point_cloud(self, h, lo, hi, step, iterations=0) -> np.ndarray:
         f3 = [np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float32), (3,))) for v in (lo, hi, step)]

@@ -21,15 +21,21 @@
 //                      tangerine/export.cpp:401-427, SDFOctree::Eval(Exact)
 //   point cloud     -> the steps of PointCloudExportThread (export.cpp:384-480;
 //                      export.cpp itself needs SDL headers) over the whole tree
+//   pivot_towards   -> the steps of LuaPivotTowards (lua_sdf.cpp:452-559; the
+//                      Lua sources need Lua headers) over the reference tree
+//   pick            -> DeliverMouseButton's nearest hit over SDFModel::RayMarch
+//                      (sdf_model.cpp:150-206, 259-264) with Transform
 
 #include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <thread>
 #include <vector>
 
 #include "sdf_evaluator.h"
+#include "transform.h"
 #include "surface_nets.h"
 #include "threadpool.h"
 
@@ -313,6 +319,134 @@ int64_t tgref_point_cloud(void* A, const float* Lo, const float* Hi, const float
 		Out[3 * i] = Vertices[i].x; Out[3 * i + 1] = Vertices[i].y; Out[3 * i + 2] = Vertices[i].z;
 	}
 	return int64_t(Vertices.size());
+}
+
+// LuaPivotTowards (tangerine/lua_sdf.cpp:452-559) over the reference tree:
+// the same glm steps, Lua argument handling aside.  Returns 0, or 2 when the
+// heading is antiparallel to down (the reference then draws random axes from
+// the Lua environment's generator; nothing is written).
+int tgref_pivot_towards(void* A, float MaxDist, float Margin, float MaxAngleDegrees, const float* PivotIn,
+	const float* HeadingIn, const float* DownIn, float* TailOut, float* HeadingOut)
+{
+	SDFNode* Node = ((Handle*)A)->Node.get();
+	const float MaxAngle = glm::radians(MaxAngleDegrees);
+	glm::vec3 Pivot(PivotIn[0], PivotIn[1], PivotIn[2]);
+	glm::vec3 Heading(HeadingIn[0], HeadingIn[1], HeadingIn[2]);
+	glm::vec3 Down(DownIn[0], DownIn[1], DownIn[2]);
+	glm::vec3 Tail;
+	glm::vec3 Axis;
+	{
+		float HeadingLen = glm::length(Heading);
+		if (HeadingLen == 0.0 || glm::isnan(HeadingLen) || glm::isinf(HeadingLen))
+		{
+			Tail = Pivot;
+			Heading = glm::vec3(0.0, 0.0, 0.0);
+			goto finish;
+		}
+		else
+		{
+			Heading /= HeadingLen;
+		}
+	}
+	{
+		float DownLen = glm::length(Down);
+		if (DownLen == 0.0 || glm::isnan(DownLen) || glm::isinf(DownLen))
+		{
+			Down = glm::vec3(0, 0, -1);
+		}
+		else
+		{
+			Down /= DownLen;
+		}
+	}
+	Tail = Heading * MaxDist + Pivot;
+	Axis = glm::cross(Heading, Down);
+	{
+		float AxisLen = glm::length(Axis);
+		if (AxisLen == 0.0)
+		{
+			if (glm::distance(Down, Heading) <= 0.001)
+			{
+				Heading = Down;
+				goto finish;
+			}
+			return 2;
+		}
+	}
+	Axis = glm::normalize(Axis);
+	{
+		float Dist;
+		float RemainingMaxAngle = MaxAngle;
+		const float Fnord = MaxDist * MaxDist * 2;
+		while (true)
+		{
+			Dist = Node->Eval(Tail);
+			if (Dist <= Margin)
+			{
+				break;
+			}
+			Dist = glm::min(glm::abs(Dist), MaxDist);
+			float Cosine = glm::abs((Fnord - Dist * Dist) / Fnord);
+			float MaybeAngle = glm::acos(Cosine);
+			float MaxDownward = glm::abs(glm::acos(glm::dot(glm::normalize(Tail - Pivot), Down)));
+			float Angle = glm::min(RemainingMaxAngle, glm::min(MaxDownward, MaybeAngle));
+			RemainingMaxAngle -= Angle;
+			glm::quat Rotation = glm::angleAxis(Angle, Axis);
+			Tail = glm::rotate(Rotation, Tail - Pivot) + Pivot;
+			if (Angle <= glm::radians(0.5))
+			{
+				break;
+			}
+		}
+		const float Cosine = glm::min(glm::abs(glm::dot(glm::normalize(Tail - Pivot), Heading)), 1.0f);
+		const float MaybeAngle = glm::acos(Cosine);
+		const float FinalAngle = glm::min(MaxAngle, MaybeAngle);
+		glm::quat Rotation = glm::angleAxis(FinalAngle, Axis);
+		Heading = glm::rotate(Rotation, Heading);
+		Tail = Heading * MaxDist + Pivot;
+	}
+finish:
+	for (int i = 0; i < 3; ++i)
+	{
+		TailOut[i] = Tail[i];
+		HeadingOut[i] = Heading[i];
+	}
+	return 0;
+}
+
+// Mouse picking (DeliverMouseButton, tangerine/sdf_model.cpp:150-206, with
+// SDFModel::RayMarch, :259-264): every model's ray in its local frame
+// (Transform::ApplyInv and the inverse rotation), RayMarch with 1000 steps
+// and the default epsilon; the nearest hit by local travel wins (the first
+// on ties).  Transforms: 8 floats each, rotation (x, y, z, w), translation,
+// scale.  Returns the model index or -1; travel and the local position.
+int tgref_pick(void* const* Models, const float* Transforms, uint32_t Count, const float* Start, const float* Dir,
+	float* TravelOut, float* PositionOut)
+{
+	float Nearest = std::numeric_limits<float>::infinity();
+	int Match = -1;
+	const glm::vec3 RayStart(Start[0], Start[1], Start[2]), RayDir(Dir[0], Dir[1], Dir[2]);
+	for (uint32_t m = 0; m < Count; ++m)
+	{
+		const float* T = Transforms + 8 * m;
+		Transform LocalToWorld;
+		LocalToWorld.Rotation = glm::quat(T[3], T[0], T[1], T[2]);
+		LocalToWorld.Translation = glm::vec3(T[4], T[5], T[6]);
+		LocalToWorld.Scalation = T[7];
+		glm::vec3 RelativeOrigin = LocalToWorld.ApplyInv(RayStart);
+		glm::vec3 RelativeRayDir = glm::rotate(glm::inverse(LocalToWorld.Rotation), RayDir);
+		RayHit Query = ((Handle*)Models[m])->Node->RayMarch(RelativeOrigin, RelativeRayDir, 1000);
+		if (Query.Hit && Query.Travel < Nearest)
+		{
+			Nearest = Query.Travel;
+			Match = int(m);
+			PositionOut[0] = Query.Position.x;
+			PositionOut[1] = Query.Position.y;
+			PositionOut[2] = Query.Position.z;
+		}
+	}
+	*TravelOut = Nearest;
+	return Match;
 }
 
 // Serial reference surface nets over precomputed sample values (identical-grid

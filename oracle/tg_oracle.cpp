@@ -1098,6 +1098,128 @@ int tgo_raymarch_far_words(const uint32_t* words, uint64_t nwords, const float* 
 	return 0;
 }
 
+// LuaPivotTowards (lua_sdf.cpp:452-559) over the interpreter: the glm
+// steps written out (radians in float, the 0.5 degree and 0.001 tests in
+// double as the reference's double literals make them).  Returns 0, 2 when
+// the heading is antiparallel to down (the reference draws random axes; no
+// output), 3 when `max_iterations` turns of the loop did not end it, or -1
+// for a malformed program.
+int tgo_pivot_towards_words(const uint32_t* words, uint64_t nwords, float max_dist, float margin, float max_angle_deg,
+	const float* pivot_in, const float* heading_in, const float* down_in, int max_iterations, float* tail_out,
+	float* heading_out)
+{
+	const float deg = 0.01745329251994329576923690768489f;
+	const float max_angle = max_angle_deg * deg;
+	const V3 pivot = { pivot_in[0], pivot_in[1], pivot_in[2] };
+	V3 heading = { heading_in[0], heading_in[1], heading_in[2] };
+	V3 down = { down_in[0], down_in[1], down_in[2] };
+	V3 tail, axis;
+	bool ok = true;
+	const float hl = len3(heading);
+	if (hl == 0.0f || std::isnan(hl) || std::isinf(hl))
+	{
+		tail = pivot;
+		heading = { 0.0f, 0.0f, 0.0f };
+	}
+	else
+	{
+		heading = div3s(heading, hl);
+		const float dl = len3(down);
+		if (dl == 0.0f || std::isnan(dl) || std::isinf(dl))
+			down = { 0.0f, 0.0f, -1.0f };
+		else
+			down = div3s(down, dl);
+		tail = add3(mul3s(heading, max_dist), pivot);
+		axis = cross3(heading, down);
+		if (len3(axis) == 0.0f)
+		{
+			if (double(len3(sub3(heading, down))) <= 0.001)
+				heading = down;
+			else
+				return 2;
+		}
+		else
+		{
+			axis = mul3s(axis, 1.0f / std::sqrt(dot3(axis, axis)));
+			float remaining = max_angle;
+			const float fnord = max_dist * max_dist * 2.0f;
+			int it = 0;
+			for (;; ++it)
+			{
+				if (it >= max_iterations) return 3;
+				float dist = interpret(words, nwords, tail, &ok);
+				if (!ok) return -1;
+				if (dist <= margin) break;
+				dist = gmin(std::fabs(dist), max_dist);
+				const float cosine = std::fabs((fnord - dist * dist) / fnord);
+				const float maybe = std::acos(cosine);
+				const V3 arm = sub3(tail, pivot);
+				const float downward = std::fabs(std::acos(dot3(mul3s(arm, 1.0f / std::sqrt(dot3(arm, arm))), down)));
+				const float angle = gmin(remaining, gmin(downward, maybe));
+				remaining -= angle;
+				const float s = std::sin(angle * 0.5f);
+				const Q4 rot = { std::cos(angle * 0.5f), axis.x * s, axis.y * s, axis.z * s };
+				tail = add3(qrotate(rot, sub3(tail, pivot)), pivot);
+				if (double(angle) <= 0.5 * 0.01745329251994329576923690768489) break;
+			}
+			const V3 arm = sub3(tail, pivot);
+			const float cosine = gmin(std::fabs(dot3(mul3s(arm, 1.0f / std::sqrt(dot3(arm, arm))), heading)), 1.0f);
+			const float final_angle = gmin(max_angle, std::acos(cosine));
+			const float s = std::sin(final_angle * 0.5f);
+			const Q4 rot = { std::cos(final_angle * 0.5f), axis.x * s, axis.y * s, axis.z * s };
+			heading = qrotate(rot, heading);
+			tail = add3(mul3s(heading, max_dist), pivot);
+		}
+	}
+	tail_out[0] = tail.x; tail_out[1] = tail.y; tail_out[2] = tail.z;
+	heading_out[0] = heading.x; heading_out[1] = heading.y; heading_out[2] = heading.z;
+	return 0;
+}
+
+// Picking (DeliverMouseButton with SDFModel::RayMarch, sdf_model.cpp:150-206,
+// 259-264) over the interpreter: model m's program is words[offsets[m] ..
+// offsets[m + 1]); transforms are 8 floats, rotation (x, y, z, w),
+// translation, scale.  Local ray: rotate(inverse(q), start - t) / s and
+// rotate(inverse(q), dir); RayMarch with 1000 steps, epsilon 0.001; nearest
+// hit by local travel, the first on ties.  Returns the model or -1.
+int tgo_pick_words(const uint32_t* words, const uint64_t* offsets, const float* xf, uint32_t count, const float* start,
+	const float* dir, float* travel_out, float* position_out)
+{
+	float nearest = std::numeric_limits<float>::infinity();
+	int match = -1;
+	const V3 s0 = { start[0], start[1], start[2] }, d0 = { dir[0], dir[1], dir[2] };
+	for (uint32_t m = 0; m < count; ++m)
+	{
+		const float* t = xf + 8 * m;
+		const Q4 inv = qinverse(Q4{ t[3], t[0], t[1], t[2] });
+		const V3 s = div3s(qrotate(inv, sub3(s0, V3{ t[4], t[5], t[6] })), t[7]);
+		V3 d = qrotate(inv, d0);
+		d = mul3s(d, 1.0f / std::sqrt(dot3(d, d)));
+		const uint32_t* w = words + offsets[m];
+		const uint64_t nw = offsets[m + 1] - offsets[m];
+		V3 ps = s;
+		float tr = 0.0f;
+		bool h = false;
+		for (int it = 0; it < 1000; ++it)
+		{
+			bool ok;
+			const float dist = interpret(w, nw, ps, &ok);
+			if (!ok) return -2;
+			if (dist <= 0.001f) { h = true; break; }
+			tr += dist;
+			ps = add3(mul3s(d, tr), s);
+		}
+		if (h && tr < nearest)
+		{
+			nearest = tr;
+			match = int(m);
+			position_out[0] = ps.x; position_out[1] = ps.y; position_out[2] = ps.z;
+		}
+	}
+	*travel_out = nearest;
+	return match;
+}
+
 // ---------------------------------------------------------------------------
 // Serial naive surface nets (surface_nets.cpp:30-580) over sample values.
 // Values: (sx+1)(sy+1)(sz+1), x fastest.  Vertex ids in raster order of the

@@ -214,6 +214,22 @@ struct CloudArgs
 	uint32_t slots;
 };
 
+// LuaPivotTowards over a batch of queries (one thread each).
+struct PivotArgs
+{
+	const Instr* prog;
+	const float* pivot;        // 3 per query
+	const float* heading;      // 3 per query
+	const float* down;         // 3 per query
+	uint64_t count;
+	float max_dist, margin, max_angle;  // max_angle in radians
+	int max_iterations;        // loop turns before status 3
+	float* tail_out;           // 3 per query
+	float* heading_out;        // 3 per query
+	int32_t* status_out;       // 0 ok, 2 heading antiparallel to down, 3 iteration cap
+	uint32_t slots;
+};
+
 // Central-difference or tetrahedral gradients at points.
 struct GradArgs
 {
@@ -230,6 +246,7 @@ struct GradArgs
 // fast = contracted FP32).  `p` is points per lane for the tile kernel (4 or 8).
 namespace exact {
 hipError_t launch_points(const PointArgs& a, hipStream_t s);
+hipError_t launch_pivot(const PivotArgs& a, hipStream_t s);
 hipError_t launch_cloud(const CloudArgs& a, int stage, hipStream_t s);  // 0 select, 1 points, 2 refine
 hipError_t launch_batch(const BatchArgs& b, int p, uint32_t waves, uint32_t workgroups, hipStream_t s);
 hipError_t launch_batch_reset(unsigned int* head, unsigned long long* work, uint32_t nwork, hipStream_t s);
@@ -242,6 +259,7 @@ hipError_t launch_gradient(const GradArgs& a, hipStream_t s);
 }
 namespace fast {
 hipError_t launch_points(const PointArgs& a, hipStream_t s);
+hipError_t launch_pivot(const PivotArgs& a, hipStream_t s);
 hipError_t launch_cloud(const CloudArgs& a, int stage, hipStream_t s);  // 0 select, 1 points, 2 refine
 hipError_t launch_batch(const BatchArgs& b, int p, uint32_t waves, uint32_t workgroups, hipStream_t s);
 hipError_t launch_batch_reset(unsigned int* head, unsigned long long* work, uint32_t nwork, hipStream_t s);

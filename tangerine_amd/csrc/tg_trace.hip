@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "../../include/tangerine_gpu.h"
 #include "tg_capi_internal.h"
@@ -24,6 +25,12 @@ using namespace tg;
 		hipError_t _e = (expr);                          \
 		if (_e != hipSuccess) return hip_fail(_e, what); \
 	} while (0)
+
+// A model transform as passed to a kernel: rotation (x, y, z, w), translation, scale.
+struct Transform8
+{
+	float x, y, z, w, tx, ty, tz, s;
+};
 
 struct tg_tracer
 {
@@ -312,5 +319,185 @@ extern "C" int tg_gradient_points(const tg_program* p, const float* xyz, uint64_
 	hipError_t e = (flags & TG_FLAG_FAST) ? fast::launch_gradient(ga, nullptr) : exact::launch_gradient(ga, nullptr);
 	if (e != hipSuccess) return hip_fail(e, "gradient launch");
 	TG_CHECK_HIP(hipMemcpy(out, dout.ptr, count * 12, hipMemcpyDeviceToHost), "gradient download");
+	return TG_OK;
+}
+
+// ---- Lua queries: pivot_towards and mouse picking --------------------------------
+
+extern "C" int tg_pivot_towards(const tg_program* p, uint64_t count, const float* pivots, const float* headings,
+	const float* downs, float max_dist, float margin, float max_angle_degrees, int max_iterations, float* tails,
+	float* headings_out, int32_t* status, uint32_t flags)
+{
+	if (!p || (count && (!pivots || !headings || !downs || !tails || !headings_out || !status)))
+		return fail(TG_ERR_INVALID, "tg_pivot_towards: null argument");
+	if (max_iterations < 1) return fail(TG_ERR_INVALID, "tg_pivot_towards: max_iterations must be positive");
+	if (count == 0) return TG_OK;
+	if (count >= (1ull << 31)) return fail(TG_ERR_LIMIT, "tg_pivot_towards: too many queries");
+	int rc = ensure_device();
+	if (rc) return rc;
+	DevBuf din, dout, dst;
+	struct Guard { DevBuf* b[3]; ~Guard() { for (DevBuf* x : b) x->release(); } } guard{ { &din, &dout, &dst } };
+	rc = din.ensure(count * 36);
+	if (!rc) rc = dout.ensure(count * 24);
+	if (!rc) rc = dst.ensure(count * 4);
+	if (rc) return rc;
+	float* in = din.as<float>();
+	TG_CHECK_HIP(hipMemcpy(in, pivots, count * 12, hipMemcpyHostToDevice), "query upload");
+	TG_CHECK_HIP(hipMemcpy(in + 3 * count, headings, count * 12, hipMemcpyHostToDevice), "query upload");
+	TG_CHECK_HIP(hipMemcpy(in + 6 * count, downs, count * 12, hipMemcpyHostToDevice), "query upload");
+	PivotArgs a{};
+	a.prog = static_cast<const Instr*>(p->device.ptr);
+	a.pivot = in;
+	a.heading = in + 3 * count;
+	a.down = in + 6 * count;
+	a.count = count;
+	a.max_dist = max_dist;
+	a.margin = margin;
+	a.max_angle = max_angle_degrees * 0.01745329251994329576923690768489f;  // glm::radians(float)
+	a.max_iterations = max_iterations;
+	a.tail_out = dout.as<float>();
+	a.heading_out = dout.as<float>() + 3 * count;
+	a.status_out = dst.as<int32_t>();
+	a.slots = std::max<uint32_t>(1, p->decoded.max_stack - 1);
+	hipError_t e = (flags & TG_FLAG_FAST) ? fast::launch_pivot(a, nullptr) : exact::launch_pivot(a, nullptr);
+	if (e != hipSuccess) return hip_fail(e, "pivot launch");
+	std::vector<float> out(size_t(count) * 6);
+	TG_CHECK_HIP(hipMemcpy(out.data(), dout.ptr, count * 24, hipMemcpyDeviceToHost), "pivot download");
+	TG_CHECK_HIP(hipMemcpy(status, dst.ptr, count * 4, hipMemcpyDeviceToHost), "pivot download");
+	for (uint64_t i = 0; i < count; ++i)
+	{
+		if (status[i] != 0) continue;  // outputs left as the caller had them
+		for (int k = 0; k < 3; ++k)
+		{
+			tails[3 * i + k] = out[3 * i + k];
+			headings_out[3 * i + k] = out[3 * (count + i) + k];
+		}
+	}
+	return TG_OK;
+}
+
+namespace {
+
+// Every ray in model space: Transform::ApplyInv for the start,
+// rotate(inverse(q), dir) for the direction (transform.cpp:64-67,
+// sdf_model.cpp:259-264).  xf: rotation (x, y, z, w), translation, scale.
+__global__ void __launch_bounds__(256) k_local_rays(const float* starts, const float* dirs, uint64_t n, Transform8 xf,
+	float* out_starts, float* out_dirs)
+{
+	const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	// inverse(q) = conjugate(q) / dot(q, q), dot = (w w + x x) + (y y + z z)
+	const float qd = (xf.w * xf.w + xf.x * xf.x) + (xf.y * xf.y + xf.z * xf.z);
+	const float w = xf.w / qd, qx = -xf.x / qd, qy = -xf.y / qd, qz = -xf.z / qd;
+	auto rot = [&](float& x, float& y, float& z) {
+		const float uvx = qy * z - y * qz, uvy = qz * x - z * qx, uvz = qx * y - x * qy;
+		const float uuvx = qy * uvz - uvy * qz, uuvy = qz * uvx - uvz * qx, uuvz = qx * uvy - uvx * qy;
+		x = x + (uvx * w + uuvx) * 2.0f;
+		y = y + (uvy * w + uuvy) * 2.0f;
+		z = z + (uvz * w + uuvz) * 2.0f;
+	};
+	float sx = starts[3 * i] - xf.tx, sy = starts[3 * i + 1] - xf.ty, sz = starts[3 * i + 2] - xf.tz;
+	rot(sx, sy, sz);
+	out_starts[3 * i] = sx / xf.s;
+	out_starts[3 * i + 1] = sy / xf.s;
+	out_starts[3 * i + 2] = sz / xf.s;
+	float dx = dirs[3 * i], dy = dirs[3 * i + 1], dz = dirs[3 * i + 2];
+	rot(dx, dy, dz);
+	out_dirs[3 * i] = dx;
+	out_dirs[3 * i + 1] = dy;
+	out_dirs[3 * i + 2] = dz;
+}
+
+// Keeps, per ray, the hit with the smallest local travel (strictly smaller:
+// the earlier model wins ties, as DeliverMouseButton's `Travel < Nearest`).
+__global__ void __launch_bounds__(256) k_pick_reduce(const uint8_t* hit, const float* travel, const float* pos,
+	uint64_t n, int32_t model, int32_t* best_model, float* best_travel, float* best_pos)
+{
+	const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	if (hit[i] && travel[i] < best_travel[i])
+	{
+		best_travel[i] = travel[i];
+		best_model[i] = model;
+		best_pos[3 * i] = pos[3 * i];
+		best_pos[3 * i + 1] = pos[3 * i + 1];
+		best_pos[3 * i + 2] = pos[3 * i + 2];
+	}
+}
+
+__global__ void __launch_bounds__(256) k_pick_init(uint64_t n, int32_t* best_model, float* best_travel, float* best_pos)
+{
+	const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	best_model[i] = -1;
+	best_travel[i] = __builtin_huge_valf();
+	best_pos[3 * i] = best_pos[3 * i + 1] = best_pos[3 * i + 2] = 0.0f;
+}
+
+}  // namespace
+
+extern "C" int tg_pick(const tg_program* const* programs, const float* transforms, uint32_t model_count,
+	const float* starts, const float* dirs, uint64_t ray_count, uint32_t flags, int32_t* model_out, float* travel_out,
+	float* position_out)
+{
+	if ((model_count && (!programs || !transforms)) || (ray_count && (!starts || !dirs || !model_out || !travel_out)))
+		return fail(TG_ERR_INVALID, "tg_pick: null argument");
+	for (uint32_t m = 0; m < model_count; ++m)
+		if (!programs[m]) return fail(TG_ERR_INVALID, "tg_pick: null program");
+	if (ray_count == 0) return TG_OK;
+	if (ray_count >= (1ull << 31)) return fail(TG_ERR_LIMIT, "tg_pick: too many rays");
+	int rc = ensure_device();
+	if (rc) return rc;
+	DevBuf rays, local, res, best;
+	struct Guard { DevBuf* b[4]; ~Guard() { for (DevBuf* x : b) x->release(); } } guard{ { &rays, &local, &res, &best } };
+	const uint64_t n = ray_count;
+	rc = rays.ensure(n * 24);
+	if (!rc) rc = local.ensure(n * 24);
+	if (!rc) rc = res.ensure(n * 17);    // travel, position, hit
+	if (!rc) rc = best.ensure(n * 20);   // model, travel, position
+	if (rc) return rc;
+	float* ray_s = rays.as<float>();
+	float* ray_d = ray_s + 3 * n;
+	float* loc_s = local.as<float>();
+	float* loc_d = loc_s + 3 * n;
+	float* r_t = res.as<float>();
+	float* r_p = r_t + n;
+	uint8_t* r_h = reinterpret_cast<uint8_t*>(r_p + 3 * n);
+	int32_t* b_m = best.as<int32_t>();
+	float* b_t = reinterpret_cast<float*>(b_m + n);
+	float* b_p = b_t + n;
+	TG_CHECK_HIP(hipMemcpy(ray_s, starts, n * 12, hipMemcpyHostToDevice), "ray upload");
+	TG_CHECK_HIP(hipMemcpy(ray_d, dirs, n * 12, hipMemcpyHostToDevice), "ray upload");
+	const unsigned blocks = unsigned((n + 255) / 256);
+	hipLaunchKernelGGL(k_pick_init, dim3(blocks), dim3(256), 0, nullptr, n, b_m, b_t, b_p);
+	TG_CHECK_HIP(hipGetLastError(), "pick init");
+	for (uint32_t m = 0; m < model_count; ++m)
+	{
+		const float* t = transforms + 8 * size_t(m);
+		if (!(t[7] != 0.0f)) return fail(TG_ERR_INVALID, "tg_pick: zero scale");
+		const Transform8 xf{ t[0], t[1], t[2], t[3], t[4], t[5], t[6], t[7] };
+		hipLaunchKernelGGL(k_local_rays, dim3(blocks), dim3(256), 0, nullptr, ray_s, ray_d, n, xf, loc_s, loc_d);
+		TG_CHECK_HIP(hipGetLastError(), "local rays");
+		TraceArgs ta;
+		base_args(ta, programs[m]);
+		ta.camera = 0;
+		ta.starts = loc_s;
+		ta.dirs = loc_d;
+		ta.nrays = uint32_t(n);
+		ta.max_steps = 1000;  // SDFModel::RayMarch: Evaluator->RayMarch(origin, dir, 1000), epsilon 0.001
+		ta.eps = 0.001f;
+		ta.far_ = __builtin_huge_valf();
+		ta.h = 0.0f;
+		ta.t_out = r_t;
+		ta.p_out = r_p;
+		ta.hit_out = r_h;
+		hipError_t e = (flags & TG_FLAG_FAST) ? fast::launch_trace(ta, nullptr) : exact::launch_trace(ta, nullptr);
+		if (e != hipSuccess) return hip_fail(e, "pick trace launch");
+		hipLaunchKernelGGL(k_pick_reduce, dim3(blocks), dim3(256), 0, nullptr, r_h, r_t, r_p, n, int32_t(m), b_m, b_t, b_p);
+		TG_CHECK_HIP(hipGetLastError(), "pick reduce");
+	}
+	TG_CHECK_HIP(hipMemcpy(model_out, b_m, n * 4, hipMemcpyDeviceToHost), "pick download");
+	TG_CHECK_HIP(hipMemcpy(travel_out, b_t, n * 4, hipMemcpyDeviceToHost), "pick download");
+	if (position_out) TG_CHECK_HIP(hipMemcpy(position_out, b_p, n * 12, hipMemcpyDeviceToHost), "pick download");
 	return TG_OK;
 }

@@ -175,6 +175,17 @@ class SDF:
         d = d * (np.float32(1.0) / np.sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]))
         return self.ray_cast(o, d, max_iterations, epsilon, fast)
 
+    def pivot_towards(self, max_dist, margin, max_angle, pivot, heading, down, fast: bool = False):
+        """Lua `pivot_towards(max_dist, margin, max_angle, pivot, heading, down)`
+        (lua_sdf.cpp:452-559): (tail, heading).  A heading antiparallel to down
+        raises (the reference picks a random axis from the Lua generator)."""
+        from .trace import pivot_towards
+        st, tails, heads = pivot_towards(self.program(), [_vec3((pivot,))], [_vec3((heading,))], [_vec3((down,))],
+                                         max_dist, margin, max_angle, fast=fast)
+        if st[0] != 0:
+            raise ValueError(f"pivot_towards: status {int(st[0])} (2: heading antiparallel to down)")
+        return tails[0], heads[0]
+
 
 def _info_dict(info) -> dict:
     return {

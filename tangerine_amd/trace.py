@@ -117,3 +117,40 @@ def gradient(program, points, h=1e-4, central=False, fast=False) -> np.ndarray:
                                          1 if central else 0, out.ctypes.data_as(_lib._fp),
                                          _lib.TG_FLAG_FAST if fast else 0), "tg_gradient_points")
     return out
+
+
+def pivot_towards(program, pivots, headings, downs, max_dist, margin, max_angle_deg, max_iterations=1 << 16,
+                  fast=False):
+    """Lua `pivot_towards` (lua_sdf.cpp:452-559) batched: (status, tails, headings).
+    Status 0 done, 2 heading antiparallel to down (no output), 3 iteration cap."""
+    p = np.ascontiguousarray(pivots, dtype=f32).reshape(-1, 3)
+    h = np.ascontiguousarray(headings, dtype=f32).reshape(-1, 3)
+    d = np.ascontiguousarray(downs, dtype=f32).reshape(-1, 3)
+    n = p.shape[0]
+    tails = np.zeros((n, 3), dtype=f32)
+    heads = np.zeros((n, 3), dtype=f32)
+    status = np.zeros(n, dtype=np.int32)
+    check(_lib.load().tg_pivot_towards(program.handle, n, p.ctypes.data_as(_lib._fp), h.ctypes.data_as(_lib._fp),
+                                       d.ctypes.data_as(_lib._fp), float(max_dist), float(margin),
+                                       float(max_angle_deg), int(max_iterations), tails.ctypes.data_as(_lib._fp),
+                                       heads.ctypes.data_as(_lib._fp), status.ctypes.data_as(C.POINTER(C.c_int32)),
+                                       _lib.TG_FLAG_FAST if fast else 0), "tg_pivot_towards")
+    return status, tails, heads
+
+
+def pick(programs, transforms, starts, dirs, fast=False):
+    """Mouse picking (DeliverMouseButton over SDFModel::RayMarch) for a batch of
+    rays: (model index or -1, local travel, local hit position)."""
+    progs = (C.c_void_p * max(len(programs), 1))(*[pr.handle for pr in programs])
+    xf = np.ascontiguousarray(transforms, dtype=f32).reshape(-1, 8)
+    s = np.ascontiguousarray(starts, dtype=f32).reshape(-1, 3)
+    d = np.ascontiguousarray(dirs, dtype=f32).reshape(-1, 3)
+    n = s.shape[0]
+    model = np.zeros(n, dtype=np.int32)
+    travel = np.zeros(n, dtype=f32)
+    pos = np.zeros((n, 3), dtype=f32)
+    check(_lib.load().tg_pick(C.cast(progs, C.POINTER(_lib._p)), xf.ctypes.data_as(_lib._fp), len(programs),
+                              s.ctypes.data_as(_lib._fp), d.ctypes.data_as(_lib._fp), n,
+                              _lib.TG_FLAG_FAST if fast else 0, model.ctypes.data_as(C.POINTER(C.c_int32)),
+                              travel.ctypes.data_as(_lib._fp), pos.ctypes.data_as(_lib._fp)), "tg_pick")
+    return model, travel, pos

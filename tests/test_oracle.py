@@ -368,3 +368,101 @@ def test_point_cloud_closed_forms(oracle):
     on = np.all(np.abs(radial - c) < half * (1 - 1e-4), axis=1)
     assert on.sum() > 100 and np.abs(np.linalg.norm(ref[on].astype(np.float64), axis=1) - 1.0).max() < 1e-6
     oracle.free(h)
+
+
+FLOOR = [("box", 20.0, 20.0, 0.5), ("move", 0.0, 0.0, -0.5)]   # half extents; top face at z = 0
+
+
+def _pivot_queries(seed, n):
+    rng = np.random.default_rng(seed)
+    for q in range(n):
+        down = np.float32([0, 0, -1]) if q % 2 else rng.normal(size=3).astype(np.float32)
+        yield rng.uniform(-2, 2, 3).astype(np.float32), rng.normal(size=3).astype(np.float32), down
+
+
+@pytest.mark.parametrize("name", ["config1", "floor", "rand16"])
+def test_pivot_towards_matches_reference(oracle, reference, name):
+    """LuaPivotTowards' steps over the reference tree (oracle/_ref) against the
+    oracle over the interpreter: bit-identical where the tree walk and the
+    interpreter agree (translations only); with rotations the ulp differences
+    can flip a margin test, so most (not all) queries agree."""
+    from tangerine_amd import scenes
+    spec = {"config1": scenes.config1_spec(), "floor": FLOOR, "rand16": scenes.random_csg_spec(5, 16)}[name]
+    h = reference.build(spec)
+    w = reference.compile(h)
+    same = total = 0
+    for piv, hd, dn in _pivot_queries(0, 200):
+        a = oracle.pivot_towards(w, 1.5, 0.01, 90.0, piv, hd, dn)
+        b = reference.pivot_towards(h, 1.5, 0.01, 90.0, piv, hd, dn)
+        assert a[0] == b[0]
+        total += 1
+        same += int(np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2]))
+    assert same == total if name != "rand16" else same >= 0.8 * total
+    reference.free(h)
+
+
+def test_pivot_towards_closed_forms(oracle):
+    """A leg of length 1.5 hinged 1 above a floor swings down from horizontal
+    until its tip is within the margin of the floor: tip at z in (-1e-3,
+    0.01], |tip - pivot| = 1.5, in the plane of heading and down.  Special
+    cases: a zero heading returns the pivot and a zero heading; a heading
+    antiparallel to down is status 2; a heading already down stays down."""
+    from tangerine_amd import scenes
+    h = oracle.build(FLOOR)
+    w = oracle.compile(h)
+    st, tail, head = oracle.pivot_towards(w, 1.5, 0.01, 90.0, (0, 0, 1), (1, 0, 0), (0, 0, -1))
+    assert st == 0
+    assert -1e-3 < tail[2] <= 0.01 and abs(tail[1]) < 1e-6 and tail[0] > 0
+    assert abs(np.linalg.norm(tail - np.float32([0, 0, 1])) - 1.5) < 1e-5
+    assert abs(np.linalg.norm(head) - 1.0) < 1e-6
+    # a planted mistake (the margin ignored) leaves the tip far below the floor or above it
+    st2, tail2, _ = oracle.pivot_towards(w, 1.5, -1.0, 90.0, (0, 0, 1), (1, 0, 0), (0, 0, -1))
+    assert st2 == 0 and abs(float(tail2[2] - tail[2])) > 1e-3
+    st, tail, head = oracle.pivot_towards(w, 1.5, 0.01, 90.0, (0, 0, 1), (0, 0, 0), (0, 0, -1))
+    assert st == 0 and np.array_equal(tail, np.float32([0, 0, 1])) and np.array_equal(head, np.zeros(3, np.float32))
+    assert oracle.pivot_towards(w, 1.5, 0.01, 90.0, (0, 0, 1), (0, 0, 2), (0, 0, -1))[0] == 2
+    st, tail, head = oracle.pivot_towards(w, 1.5, 0.01, 90.0, (0, 0, 3), (0, 0, -2), (0, 0, -1))
+    assert st == 0 and np.array_equal(head, np.float32([0, 0, -1])) and np.allclose(tail, [0, 0, 1.5])
+    oracle.free(h)
+
+
+PICK_XF = [[0, 0, 0, 1, 0, 0, 0, 1], [0.0, 0.3826834, 0.0, 0.9238795, 1.0, 0.5, 0.0, 1.5],
+           [0.0, 0.0, 0.0, 1.0, -2.0, 0.0, 0.5, 0.5]]
+
+
+def test_pick_matches_reference(oracle, reference):
+    """DeliverMouseButton's nearest hit over SDFModel::RayMarch: the oracle
+    (interpreter) against the reference tree and Transform, bit for bit on
+    translation-only trees under rotated and scaled model transforms."""
+    from tangerine_amd import scenes
+    specs = [scenes.config1_spec(), FLOOR, [("sphere", 1.0), ("move", 0.0, 0.0, 1.0)]]
+    hs = [reference.build(s) for s in specs]
+    ws = [reference.compile(h) for h in hs]
+    rng = np.random.default_rng(3)
+    hits = 0
+    for _ in range(200):
+        s = rng.uniform(-6, 6, 3).astype(np.float32)
+        d = (-s + rng.normal(size=3) * 0.5).astype(np.float32)
+        a = oracle.pick(ws, PICK_XF, s, d)
+        b = reference.pick(hs, PICK_XF, s, d)
+        assert a[0] == b[0] and np.float32(a[1]).tobytes() == np.float32(b[1]).tobytes()
+        if a[0] >= 0:
+            hits += 1
+            assert np.array_equal(a[2], b[2])
+    assert hits > 50
+    for h in hs:
+        reference.free(h)
+
+
+def test_pick_closed_form(oracle):
+    """A unit sphere scaled by 2 and moved to x = 5, seen along +x from the
+    origin: hit at world x = 3 - epsilon-ish, local travel (3 / 2) minus the
+    march's last step, local hit on the sphere; a model behind the camera is
+    never picked; a planted mistake (no scale) changes the travel."""
+    w = oracle.compile(oracle.build([("sphere", 1.0)]))
+    xf = [[0, 0, 0, 1, 5.0, 0, 0, 2.0], [0, 0, 0, 1, -5.0, 0, 0, 1.0]]
+    m, t, pos = oracle.pick([w, w], xf, (0, 0, 0), (1, 0, 0))
+    assert m == 0 and abs(t - 1.5) <= 1e-3 and abs(float(np.linalg.norm(pos)) - 1.0) <= 1e-3
+    m2, t2, _ = oracle.pick([w, w], [[0, 0, 0, 1, 5.0, 0, 0, 1.0], xf[1]], (0, 0, 0), (1, 0, 0))
+    assert m2 == 0 and abs(t2 - 4.0) <= 1e-3
+    assert oracle.pick([w], [xf[1]], (0, 0, 0), (1, 0, 0))[0] == -1
