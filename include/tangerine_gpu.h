@@ -297,6 +297,24 @@ int tg_mesher_grid_get(const tg_mesher* mesher, tg_grid* out);
 int tg_mesher_export_points(tg_mesher* mesher, const tg_program* program, const float lo[3], const float hi[3],
                             const float step[3], int refine_iterations, uint32_t flags);
 
+/* ---- MagicaVoxel export (VoxExport, magica.cpp:27-70) -------------------------
+ * A grid of ivec3(ceil(max - min) * grid_size) voxels over `bounds` (min xyz,
+ * max xyz: the model's SDFNode::Bounds); voxel (x, y, z) is kept when the
+ * program's |distance| at mix(min, max, (x + .5, y + .5, z + .5) / size) is at
+ * most the distance from min to mix(min, max, 0.5 / size).  When `path` is
+ * not NULL the voxels are written as the .vox file the reference's
+ * vox::VoxWriter writes for them added in raster order (colour index
+ * |color_index| % 255 + 1, no palette); the reference adds them in thread
+ * order.  voxel_count (may be NULL) receives the count;
+ * tg_mesher_copy_vox_cells copies the kept cell indices x + size.x (y +
+ * size.y z), raster order.  At most 2^31 - 1 cells. */
+int tg_mesher_export_vox(tg_mesher* mesher, const tg_program* program, const float bounds[6], float grid_size,
+                         int color_index, const char* path, uint32_t flags, uint64_t* voxel_count);
+int tg_mesher_copy_vox_cells(const tg_mesher* mesher, uint32_t* cells);
+/* The same .vox writer for a caller's cells (indices x + size.x (y + size.y
+ * z), in the order vox::VoxWriter::AddVoxel would receive them; host only). */
+int tg_vox_write(const char* path, const uint32_t size[3], const uint32_t* cells, uint64_t count, int color_index);
+
 /* ---- Sodapop meshing job (MeshingJob::NaiveSurfaceNets, sodapop.cpp:562-897) --
  * The interactive mesher's grid (NaiveSurfaceNetsScratch, sodapop.cpp:156-177):
  * the bounding cube of `bounds` (min xyz, max xyz: the model's SDFNode::Bounds),

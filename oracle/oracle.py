@@ -274,6 +274,24 @@ class Oracle(_Cpu):
                              _fptr(d), _fptr(tr), _fptr(pos))
         return m, float(tr[0]), pos
 
+    def vox_cells(self, words, bounds, grid_size):
+        """VoxExport's selection over the interpreter: (size, raster cell indices)."""
+        w = np.ascontiguousarray(words, dtype=np.uint32)
+        b = np.asarray(bounds, dtype=np.float32).reshape(-1)
+        mn, mx = np.ascontiguousarray(b[:3]), np.ascontiguousarray(b[3:])
+        size = np.zeros(3, dtype=np.int32)
+        L = self.lib
+        L.tgo_vox_cells.restype = C.c_int64
+        L.tgo_vox_cells.argtypes = [_u32p, _u64, _fp, _fp, _f, C.POINTER(C.c_int32), _u32p, _u64]
+        n = L.tgo_vox_cells(w.ctypes.data_as(_u32p), w.size, _fptr(mn), _fptr(mx), float(grid_size),
+                            size.ctypes.data_as(C.POINTER(C.c_int32)), None, 0)
+        if n < 0:
+            return size, np.zeros(0, dtype=np.uint32)
+        out = np.zeros(n, dtype=np.uint32)
+        L.tgo_vox_cells(w.ctypes.data_as(_u32p), w.size, _fptr(mn), _fptr(mx), float(grid_size),
+                        size.ctypes.data_as(C.POINTER(C.c_int32)), out.ctypes.data_as(_u32p), n)
+        return size, out
+
     def mt19937_10000th(self) -> int:
         self.lib.tgo_mt19937_10000th.restype = C.c_uint32
         return int(self.lib.tgo_mt19937_10000th())
@@ -333,6 +351,20 @@ class Reference(_Cpu):
         L.tgref_pick.argtypes = [C.c_void_p, _fp, C.c_uint32, _fp, _fp, _fp, _fp]
         m = L.tgref_pick(C.cast(hs, C.c_void_p), _fptr(xf), len(handles), _fptr(s), _fptr(d), _fptr(tr), _fptr(pos))
         return m, float(tr[0]), pos
+
+    def vox_write(self, size, cells, color_index, path) -> None:
+        s = np.ascontiguousarray(size, dtype=np.uint32).reshape(3)
+        c = np.ascontiguousarray(cells, dtype=np.uint32).reshape(-1)
+        L = self.lib
+        L.tgref_vox_write.restype = None
+        L.tgref_vox_write.argtypes = [_u32p, _u32p, _u64, C.c_int, C.c_char_p]
+        L.tgref_vox_write(s.ctypes.data_as(_u32p), c.ctypes.data_as(_u32p), c.size, int(color_index), str(path).encode())
+
+    def vox_export(self, h, grid_size, color_index, path) -> int:
+        L = self.lib
+        L.tgref_vox_export.restype = C.c_int64
+        L.tgref_vox_export.argtypes = [_p, _f, C.c_int, C.c_char_p]
+        return int(L.tgref_vox_export(h, float(grid_size), int(color_index), str(path).encode()))
 
     def point_cloud(self, h, lo, hi, step, iterations=0) -> np.ndarray:
         f3 = [np.ascontiguousarray(np.broadcast_to(np.asarray(v, dtype=np.float32), (3,))) for v in (lo, hi, step)]

@@ -25,6 +25,8 @@
 //                      Lua sources need Lua headers) over the reference tree
 //   pick            -> DeliverMouseButton's nearest hit over SDFModel::RayMarch
 //                      (sdf_model.cpp:150-206, 259-264) with Transform
+//   vox export      -> the steps of VoxExport (magica.cpp:27-70) serially,
+//                      written by third_party/voxwriter's vox::VoxWriter
 
 #include <atomic>
 #include <cmath>
@@ -38,6 +40,7 @@
 #include "transform.h"
 #include "surface_nets.h"
 #include "threadpool.h"
+#include "VoxWriter.h"
 
 using namespace glm;
 
@@ -447,6 +450,62 @@ int tgref_pick(void* const* Models, const float* Transforms, uint32_t Count, con
 	}
 	*TravelOut = Nearest;
 	return Match;
+}
+
+// VoxExport (tangerine/magica.cpp:27-70; magica.cpp needs the GUI's extern.h)
+// over the reference tree, serially in raster order, written by the
+// reference's own vox::VoxWriter.  Returns the voxel count, or -1 if the
+// grid is empty.
+int64_t tgref_vox_export(void* A, float GridSize, int ColorIndex, const char* Path)
+{
+	SDFNodeShared Evaluator = ((Handle*)A)->Node;
+	AABB Bounds = Evaluator->Bounds();
+	glm::ivec3 Size = glm::ivec3(glm::ceil(Bounds.Max - Bounds.Min) * GridSize);
+	if (Size.x <= 0 || Size.y <= 0 || Size.z <= 0)
+	{
+		return -1;
+	}
+	float Radius;
+	{
+		glm::vec3 Alpha = glm::vec3(.5, .5, .5) / glm::vec3(Size);
+		Radius = glm::distance(Bounds.Min, glm::mix(Bounds.Min, Bounds.Max, Alpha));
+	}
+	const int Slice = Size.x * Size.y;
+	const int TotalCells = Size.x * Size.y * Size.z;
+	vox::VoxWriter Writer(Size.x, Size.y, Size.z);
+	int64_t Count = 0;
+	for (int i = 0; i < TotalCells; ++i)
+	{
+		int z = i / Slice;
+		int y = (i % Slice) / Size.x;
+		int x = i % Size.x;
+		glm::vec3 Alpha = glm::vec3(x + .5, y + .5, z + .5) / glm::vec3(Size);
+		glm::vec3 Point = glm::mix(Bounds.Min, Bounds.Max, Alpha);
+		float Dist = Evaluator->Eval(Point);
+		if (std::abs(Dist) <= Radius)
+		{
+			Writer.AddVoxel(x, y, z, abs(ColorIndex) % 255 + 1);
+			++Count;
+		}
+	}
+	std::string PathStr(Path);
+	Writer.SaveToFile(PathStr);
+	return Count;
+}
+
+// vox::VoxWriter over a list of raster cell indices, added in the given order.
+void tgref_vox_write(const uint32_t* Size, const uint32_t* Cells, uint64_t Count, int ColorIndex, const char* Path)
+{
+	const int32_t SX = int32_t(Size[0]), SY = int32_t(Size[1]), SZ = int32_t(Size[2]);
+	vox::VoxWriter Writer(SX, SY, SZ);
+	const uint64_t Slice = uint64_t(Size[0]) * Size[1];
+	for (uint64_t i = 0; i < Count; ++i)
+	{
+		const uint64_t c = Cells[i];
+		Writer.AddVoxel(int32_t(c % Size[0]), int32_t((c % Slice) / Size[0]), int32_t(c / Slice), abs(ColorIndex) % 255 + 1);
+	}
+	std::string PathStr(Path);
+	Writer.SaveToFile(PathStr);
 }
 
 // Serial reference surface nets over precomputed sample values (identical-grid

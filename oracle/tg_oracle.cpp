@@ -1220,6 +1220,46 @@ int tgo_pick_words(const uint32_t* words, const uint64_t* offsets, const float* 
 	return match;
 }
 
+// VoxExport's voxel selection (magica.cpp:27-70) over the interpreter, in
+// raster order: Size = ivec3(ceil(max - min) * grid_size), radius = the
+// distance from min to mix(min, max, 0.5 / Size) (glm's mix, min (1 - t) +
+// max t), cell (x, y, z) kept when
+// |d(mix(min, max, (x + .5, y + .5, z + .5) / Size))| <= radius.  Writes the
+// size and up to `cap` cell indices x + Size.x (y + Size.y z); returns the
+// count, -1 for a malformed program, -2 for an empty grid.
+int64_t tgo_vox_cells(const uint32_t* words, uint64_t nwords, const float* mn, const float* mx, float grid_size,
+	int32_t* size_out, uint32_t* cells, uint64_t cap)
+{
+	int32_t size[3];
+	for (int a = 0; a < 3; ++a) size[a] = int32_t(std::ceil(mx[a] - mn[a]) * grid_size);
+	for (int a = 0; a < 3; ++a) size_out[a] = size[a];
+	if (size[0] <= 0 || size[1] <= 0 || size[2] <= 0) return -2;
+	const V3 lo = { mn[0], mn[1], mn[2] }, hi = { mx[0], mx[1], mx[2] };
+	const V3 fs = { float(size[0]), float(size[1]), float(size[2]) };
+	// glm::mix(a, b, t) = a * (1 - t) + b * t (glm detail/func_common.inl:87)
+	auto mix = [&](V3 t) {
+		return V3{ lo.x * (1.0f - t.x) + hi.x * t.x, lo.y * (1.0f - t.y) + hi.y * t.y, lo.z * (1.0f - t.z) + hi.z * t.z };
+	};
+	const V3 p0 = mix(V3{ 0.5f / fs.x, 0.5f / fs.y, 0.5f / fs.z });
+	const float radius = len3(sub3(p0, lo));
+	const int64_t slice = int64_t(size[0]) * size[1], total = slice * size[2];
+	int64_t count = 0;
+	for (int64_t i = 0; i < total; ++i)
+	{
+		const int64_t z = i / slice, y = (i % slice) / size[0], x = i % size[0];
+		const V3 p = mix(V3{ float(double(x) + .5) / fs.x, float(double(y) + .5) / fs.y, float(double(z) + .5) / fs.z });
+		bool ok;
+		const float d = interpret(words, nwords, p, &ok);
+		if (!ok) return -1;
+		if (std::fabs(d) <= radius)
+		{
+			if (uint64_t(count) < cap) cells[count] = uint32_t(i);
+			++count;
+		}
+	}
+	return count;
+}
+
 // ---------------------------------------------------------------------------
 // Serial naive surface nets (surface_nets.cpp:30-580) over sample values.
 // Values: (sx+1)(sy+1)(sz+1), x fastest.  Vertex ids in raster order of the

@@ -147,6 +147,9 @@ SIGNATURES = {
     "tg_mesher_export_sodapop": (C.c_int, [_p, _p, _fp, _f, C.c_int, _u32]),
     "tg_mesher_copy_normals": (C.c_int, [_p, _fp]),
     "tg_mesher_export_points": (C.c_int, [_p, _p, _fp, _fp, _fp, C.c_int, _u32]),
+    "tg_mesher_export_vox": (C.c_int, [_p, _p, _fp, _f, C.c_int, C.c_char_p, _u32, C.POINTER(_u64)]),
+    "tg_mesher_copy_vox_cells": (C.c_int, [_p, _u32p]),
+    "tg_vox_write": (C.c_int, [C.c_char_p, _u32p, _u32p, _u64, C.c_int]),
     "tg_batch_create": (C.c_int, [C.POINTER(_p)]),
     "tg_batch_destroy": (None, [_p]),
     "tg_batch_export": (C.c_int, [_p, C.POINTER(_p), C.POINTER(TgGrid), _u32, _u32]),

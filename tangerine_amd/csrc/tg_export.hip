@@ -6,7 +6,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
+#include <map>
 #include <random>
 #include <vector>
 #include <cmath>
@@ -36,7 +38,8 @@ struct tg_mesher
 	hipEvent_t ev[6] = {};
 	DevBuf pool, signs, active, word_count, word_base, quad_count, quad_base, vertices, triangles, scan_tmp, dense, cubes;
 	DevBuf normals, exchange;
-	DevBuf cloud_flags, cloud_rank;
+	DevBuf cloud_flags, cloud_rank, vox_list;
+	uint64_t nvox = 0;            // voxels of the last MagicaVoxel export
 	bool have_normals = false;
 	DevBuf counters;
 	DevBuf brick_flags, brick_active, tile_mask, block_list, tile_list;
@@ -1153,6 +1156,218 @@ extern "C" int tg_mesher_export_points(tg_mesher* m, const tg_program* p, const 
 	m->stats.eval_ms = ms;
 	m->stats.total_ms = ms;
 	m->stats.exact_evals = cells + np * uint64_t(refine_iterations) * 5;
+	return TG_OK;
+}
+
+// ---- MagicaVoxel export (tangerine/magica.cpp:27-70) -------------------------
+
+namespace {
+
+// The .vox file that VoxExport's vox::VoxWriter (third_party/voxwriter)
+// writes for voxels added in raster order: models ("cubes") of at most
+// min(size, 126) voxels per axis, created in the order their first voxel
+// comes; a scene graph of a root transform, a group and one transform and
+// shape per model (its "_t" translation from the writer's centring formula,
+// min-cube tracking included); no palette.  Little-endian chunks.
+struct VoxFile
+{
+	std::vector<unsigned char> b;
+	void u32(uint32_t v) { for (int k = 0; k < 4; ++k) b.push_back(uint8_t(v >> (8 * k))); }
+	void i32(int32_t v) { u32(uint32_t(v)); }
+	void id(const char* s) { b.insert(b.end(), s, s + 4); }
+	void str(const std::string& s) { u32(uint32_t(s.size())); b.insert(b.end(), s.begin(), s.end()); }
+	void chunk(const char* name, uint32_t content) { id(name); u32(content); u32(0); }
+};
+
+int write_vox(const char* path, const uint32_t size[3], const std::vector<uint32_t>& cells, uint8_t color)
+{
+	int32_t per[3];
+	for (int a = 0; a < 3; ++a) per[a] = std::min<int32_t>(std::max<int32_t>(int32_t(size[a]), 0), 126);
+	struct Cube
+	{
+		int32_t o[3];
+		std::vector<uint8_t> xyzi;
+	};
+	std::vector<Cube> cubes;
+	std::map<std::array<int32_t, 3>, size_t> index;
+	int32_t min_cube[3] = { int32_t(1e7), int32_t(1e7), int32_t(1e7) };
+	double lower[3] = { 1e7, 1e7, 1e7 }, upper[3] = { 0.0, 0.0, 0.0 };
+	const uint64_t slice = uint64_t(size[0]) * size[1];
+	for (const uint32_t c : cells)
+	{
+		const int32_t v[3] = { int32_t(c % size[0]), int32_t((c % slice) / size[0]), int32_t(c / slice) };
+		const std::array<int32_t, 3> o = { int32_t(std::floor(double(v[0]) / double(per[0]))),
+			                               int32_t(std::floor(double(v[1]) / double(per[1]))),
+			                               int32_t(std::floor(double(v[2]) / double(per[2]))) };
+		// VoxWriter::AddVoxel tracks y and z against the x minimum
+		min_cube[0] = std::min(min_cube[0], o[0]);
+		min_cube[1] = std::min(min_cube[0], o[1]);
+		min_cube[2] = std::min(min_cube[0], o[2]);
+		auto it = index.find(o);
+		if (it == index.end())
+		{
+			it = index.emplace(o, cubes.size()).first;
+			cubes.push_back(Cube{ { o[0], o[1], o[2] }, {} });
+		}
+		Cube& cube = cubes[it->second];
+		for (int a = 0; a < 3; ++a)
+		{
+			lower[a] = std::min(lower[a], double(v[a]));
+			upper[a] = std::max(upper[a], double(v[a]));
+		}
+		for (int a = 0; a < 3; ++a) cube.xyzi.push_back(uint8_t(v[a] % per[a]));
+		cube.xyzi.push_back(color);
+	}
+	VoxFile f;
+	f.id("VOX ");
+	f.i32(150);
+	f.id("MAIN");
+	f.u32(0);
+	f.u32(0);  // children bytes, patched below
+	const size_t header = f.b.size();
+	for (const Cube& c : cubes)
+	{
+		f.chunk("SIZE", 12);
+		for (int a = 0; a < 3; ++a) f.i32(per[a] + 1);
+		f.chunk("XYZI", uint32_t(4 + c.xyzi.size()));
+		f.u32(uint32_t(c.xyzi.size() / 4));
+		f.b.insert(f.b.end(), c.xyzi.begin(), c.xyzi.end());
+	}
+	const int32_t count = int32_t(cubes.size());
+	// root transform (node 0) -> group (node 1)
+	f.chunk("nTRN", 28);
+	f.i32(0); f.u32(0); f.i32(1); f.i32(-1); f.i32(-1); f.i32(1); f.u32(0);
+	f.chunk("nGRP", uint32_t(4 * (2 + count) + 4));
+	f.i32(1); f.u32(0); f.i32(count);
+	for (int32_t i = 0; i < count; ++i) f.i32(2 + 2 * i);
+	for (int32_t i = 0; i < count; ++i)
+	{
+		const Cube& c = cubes[size_t(i)];
+		int32_t t[3];
+		for (int a = 0; a < 2; ++a)
+		{
+			const float cube_centre = (float(c.o[a] - min_cube[a]) + 0.5f) * float(per[a]);
+			t[a] = int32_t(std::floor(double(cube_centre) - lower[a] - (upper[a] - lower[a]) * 0.5));
+		}
+		t[2] = int32_t(std::floor((float(c.o[2] - min_cube[2]) + 0.5f) * float(per[2])));
+		const std::string value = std::to_string(t[0]) + " " + std::to_string(t[1]) + " " + std::to_string(t[2]);
+		f.chunk("nTRN", uint32_t(20 + 4 + 4 + (4 + 2) + (4 + value.size())));
+		f.i32(2 + 2 * i); f.u32(0); f.i32(3 + 2 * i); f.i32(-1); f.i32(0); f.i32(1);
+		f.u32(1); f.str("_t"); f.str(value);
+		f.chunk("nSHP", 20);
+		f.i32(3 + 2 * i); f.u32(0); f.i32(1); f.i32(i); f.u32(0);
+	}
+	const uint32_t children = uint32_t(f.b.size() - header);
+	for (int k = 0; k < 4; ++k) f.b[header - 4 + size_t(k)] = uint8_t(children >> (8 * k));
+	File out;
+	out.f = std::fopen(path, "wb");
+	if (!out.f) return fail(TG_ERR_INVALID, std::string("tg_mesher_export_vox: cannot open ") + path);
+	if (std::fwrite(f.b.data(), 1, f.b.size(), out.f) != f.b.size())
+		return fail(TG_ERR_INVALID, std::string("tg_mesher_export_vox: write failed: ") + path);
+	return TG_OK;
+}
+
+}  // namespace
+
+extern "C" int tg_mesher_export_vox(tg_mesher* m, const tg_program* p, const float bounds[6], float grid_size,
+	int color_index, const char* path, uint32_t flags, uint64_t* voxel_count)
+{
+	if (!m || !p || !bounds) return fail(TG_ERR_INVALID, "tg_mesher_export_vox: null argument");
+	VoxArgs a{};
+	for (int i = 0; i < 3; ++i)
+	{
+		a.lo[i] = bounds[i];
+		a.hi[i] = bounds[3 + i];
+		if (!std::isfinite(a.lo[i]) || !std::isfinite(a.hi[i]))
+			return fail(TG_ERR_INVALID, "tg_mesher_export_vox: bounds must be finite");
+		// ivec3(ceil(max - min) * grid_size)
+		const float s = std::ceil(a.hi[i] - a.lo[i]) * grid_size;
+		if (!(s >= 1.0f)) return fail(TG_ERR_INVALID, "tg_mesher_export_vox: empty voxel grid");
+		if (!(s < 65536.0f)) return fail(TG_ERR_LIMIT, "tg_mesher_export_vox: more than 65535 voxels per axis");
+		a.size[i] = uint32_t(s);
+		a.fs[i] = float(a.size[i]);
+	}
+	a.cells = uint64_t(a.size[0]) * a.size[1] * a.size[2];
+	if (a.cells >= (1ull << 31)) return fail(TG_ERR_LIMIT, "tg_mesher_export_vox: more than 2^31 - 1 cells");
+	{
+		// glm::distance(min, mix(min, max, 0.5 / size))
+		float d2[3];
+		for (int i = 0; i < 3; ++i)
+		{
+			const float t = 0.5f / a.fs[i];
+			d2[i] = (a.lo[i] * (1.0f - t) + a.hi[i] * t) - a.lo[i];
+		}
+		a.radius = std::sqrt((d2[0] * d2[0] + d2[1] * d2[1]) + d2[2] * d2[2]);
+	}
+	a.prog = static_cast<const Instr*>(p->device.ptr);
+	a.slots = std::max<uint32_t>(1, p->decoded.max_stack - 1);
+	m->have_grid = false;
+	m->have_normals = false;
+	m->nbricks = 0;
+	m->nverts = m->ntris = 0;
+	m->nvox = 0;
+	m->stats = tg_export_stats{};
+	m->stats.samples = a.cells;
+	const bool fast = (flags & TG_FLAG_FAST) != 0;
+	int rc = m->cloud_flags.ensure(a.cells * 4);
+	if (!rc) rc = m->cloud_rank.ensure(a.cells * 4);
+	if (rc) return rc;
+	size_t tmp = 0;
+	TG_CHECK_HIP(exclusive_scan_u32(nullptr, nullptr, a.cells, nullptr, tmp, m->stream), "scan sizing");
+	rc = m->scan_tmp.ensure(tmp);
+	if (rc) return rc;
+	a.flags = m->cloud_flags.as<uint32_t>();
+	a.rank = m->cloud_rank.as<const uint32_t>();
+	TG_CHECK_HIP(hipEventRecord(m->ev[0], m->stream), "event");
+	hipError_t e = fast ? fast::launch_vox(a, 0, m->stream) : exact::launch_vox(a, 0, m->stream);
+	if (e != hipSuccess) return hip_fail(e, "voxel select launch");
+	tmp = m->scan_tmp.bytes;
+	TG_CHECK_HIP(exclusive_scan_u32(a.flags, m->cloud_rank.as<uint32_t>(), a.cells, m->scan_tmp.ptr, tmp, m->stream),
+		"voxel scan");
+	uint32_t tail[2] = { 0, 0 };
+	TG_CHECK_HIP(hipMemcpyAsync(&tail[0], a.rank + a.cells - 1, 4, hipMemcpyDeviceToHost, m->stream), "count copy");
+	TG_CHECK_HIP(hipMemcpyAsync(&tail[1], a.flags + a.cells - 1, 4, hipMemcpyDeviceToHost, m->stream), "count copy");
+	TG_CHECK_HIP(hipStreamSynchronize(m->stream), "voxel count");
+	const uint64_t nv = uint64_t(tail[0]) + tail[1];
+	rc = m->vox_list.ensure(std::max<uint64_t>(nv, 1) * 4);
+	if (rc) return rc;
+	a.list = m->vox_list.as<uint32_t>();
+	if (nv)
+	{
+		e = fast ? fast::launch_vox(a, 1, m->stream) : exact::launch_vox(a, 1, m->stream);
+		if (e != hipSuccess) return hip_fail(e, "voxel list launch");
+	}
+	TG_CHECK_HIP(hipEventRecord(m->ev[1], m->stream), "event");
+	std::vector<uint32_t> cells(nv);
+	if (nv) TG_CHECK_HIP(hipMemcpyAsync(cells.data(), a.list, nv * 4, hipMemcpyDeviceToHost, m->stream), "voxel download");
+	TG_CHECK_HIP(hipStreamSynchronize(m->stream), "voxels");
+	float ms = 0.0f;
+	TG_CHECK_HIP(hipEventElapsedTime(&ms, m->ev[0], m->ev[1]), "event time");
+	m->stats.eval_ms = ms;
+	m->stats.total_ms = ms;
+	m->stats.exact_evals = a.cells;
+	m->nvox = nv;
+	if (voxel_count) *voxel_count = nv;
+	if (path) return write_vox(path, a.size, cells, uint8_t(std::abs(color_index) % 255 + 1));
+	return TG_OK;
+}
+
+extern "C" int tg_vox_write(const char* path, const uint32_t size[3], const uint32_t* cells, uint64_t count,
+	int color_index)
+{
+	if (!path || !size || (count && !cells)) return fail(TG_ERR_INVALID, "tg_vox_write: null argument");
+	const uint64_t total = uint64_t(size[0]) * size[1] * size[2];
+	if (size[0] == 0 || size[1] == 0 || size[2] == 0) return fail(TG_ERR_INVALID, "tg_vox_write: empty grid");
+	for (uint64_t i = 0; i < count; ++i)
+		if (cells[i] >= total) return fail(TG_ERR_INVALID, "tg_vox_write: cell index outside the grid");
+	std::vector<uint32_t> list(cells, cells + count);
+	return write_vox(path, size, list, uint8_t(std::abs(color_index) % 255 + 1));
+}
+
+extern "C" int tg_mesher_copy_vox_cells(const tg_mesher* m, uint32_t* cells)
+{
+	if (!m || !cells) return fail(TG_ERR_INVALID, "tg_mesher_copy_vox_cells: null argument");
+	if (m->nvox) TG_CHECK_HIP(hipMemcpy(cells, m->vox_list.ptr, m->nvox * 4, hipMemcpyDeviceToHost), "voxel download");
 	return TG_OK;
 }
 

@@ -214,6 +214,22 @@ struct CloudArgs
 	uint32_t slots;
 };
 
+// VoxExport's voxel selection: cell (x, y, z) of a size[0] x size[1] x size[2]
+// grid over [lo, hi] is kept when |d| <= radius at its centre.
+struct VoxArgs
+{
+	const Instr* prog;
+	float lo[3], hi[3];
+	float fs[3];               // float(size)
+	uint32_t size[3];
+	uint64_t cells;
+	float radius;
+	uint32_t* flags;           // per cell: kept
+	const uint32_t* rank;      // exclusive scan of flags
+	uint32_t* list;            // kept cell indices, raster order
+	uint32_t slots;
+};
+
 // LuaPivotTowards over a batch of queries (one thread each).
 struct PivotArgs
 {
@@ -246,6 +262,7 @@ struct GradArgs
 // fast = contracted FP32).  `p` is points per lane for the tile kernel (4 or 8).
 namespace exact {
 hipError_t launch_points(const PointArgs& a, hipStream_t s);
+hipError_t launch_vox(const VoxArgs& a, int stage, hipStream_t s);  // 0 select, 1 list
 hipError_t launch_pivot(const PivotArgs& a, hipStream_t s);
 hipError_t launch_cloud(const CloudArgs& a, int stage, hipStream_t s);  // 0 select, 1 points, 2 refine
 hipError_t launch_batch(const BatchArgs& b, int p, uint32_t waves, uint32_t workgroups, hipStream_t s);
@@ -259,6 +276,7 @@ hipError_t launch_gradient(const GradArgs& a, hipStream_t s);
 }
 namespace fast {
 hipError_t launch_points(const PointArgs& a, hipStream_t s);
+hipError_t launch_vox(const VoxArgs& a, int stage, hipStream_t s);  // 0 select, 1 list
 hipError_t launch_pivot(const PivotArgs& a, hipStream_t s);
 hipError_t launch_cloud(const CloudArgs& a, int stage, hipStream_t s);  // 0 select, 1 points, 2 refine
 hipError_t launch_batch(const BatchArgs& b, int p, uint32_t waves, uint32_t workgroups, hipStream_t s);

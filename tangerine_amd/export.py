@@ -150,6 +150,22 @@ class Mesher:
                                                _lib.TG_FLAG_FAST if fast else 0), "tg_mesher_export_points")
         return self.stats()
 
+    def export_vox(self, program, bounds, grid_size: float, color_index: int = 0, path=None,
+                   fast: bool = False) -> np.ndarray:
+        """MagicaVoxel export (VoxExport): writes `path` when given and returns
+        the kept cell indices (raster order) of the ceil(max - min) * grid_size
+        voxel grid over `bounds` (2x3)."""
+        b = (C.c_float * 6)(*[float(v) for v in np.asarray(bounds, dtype=np.float32).reshape(-1)])
+        n = _lib._u64(0)
+        self._cells = (0, 0, 0)
+        check(self.lib.tg_mesher_export_vox(self.h, program.handle, b, float(grid_size), int(color_index),
+                                            None if path is None else str(path).encode(),
+                                            _lib.TG_FLAG_FAST if fast else 0, C.byref(n)), "tg_mesher_export_vox")
+        out = np.zeros(n.value, dtype=np.uint32)
+        if n.value:
+            check(self.lib.tg_mesher_copy_vox_cells(self.h, out.ctypes.data_as(_lib._u32p)), "tg_mesher_copy_vox_cells")
+        return out
+
     def grid_spec(self):
         """(origin, step, cells) of the last export."""
         g = _lib.TgGrid()
@@ -167,6 +183,15 @@ class Mesher:
         out = np.zeros(nx * ny * nz, dtype=np.float32)
         check(self.lib.tg_mesher_copy_grid(self.h, out.ctypes.data_as(_lib._fp)), "tg_mesher_copy_grid")
         return out
+
+
+def vox_write(path, size, cells, color_index: int = 0) -> None:
+    """Writes cells (raster indices of a size[0] x size[1] x size[2] grid) as a
+    MagicaVoxel file, in the order given (host only)."""
+    s = np.ascontiguousarray(size, dtype=np.uint32).reshape(3)
+    c = np.ascontiguousarray(cells, dtype=np.uint32).reshape(-1)
+    check(_lib.load().tg_vox_write(str(path).encode(), s.ctypes.data_as(_lib._u32p), c.ctypes.data_as(_lib._u32p),
+                                   c.size, int(color_index)), "tg_vox_write")
 
 
 def export_grid_for_bounds(bounds: np.ndarray, step: float):

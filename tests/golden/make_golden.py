@@ -151,6 +151,27 @@ def main() -> int:
         summary[f"cloud_{name}"] = {k: int(len(v)) for k, v in clouds.items()}
         ref.free(hh)
 
+    # ---- MagicaVoxel: VoxExport steps over the reference tree, and its
+    # VoxWriter over a scattered multi-model cell list ----------------------
+    import tempfile
+    with tempfile.TemporaryDirectory() as tmp:
+        spec = scenes.config1_spec()
+        hh = ref.build(spec)
+        path = Path(tmp) / "a.vox"
+        n = ref.vox_export(hh, 16.0, 3, path)
+        data = np.frombuffer(path.read_bytes(), dtype=np.uint8)
+        np.savez_compressed(OUT / "vox_config1_g16.npz", spec=spec_array(spec), bounds=ref.bounds(hh),
+                            grid_size=np.float32(16.0), color_index=np.int64(3), voxels=np.int64(n), vox_bytes=data)
+        summary["vox_config1_g16"] = {"voxels": n, "bytes": int(data.size)}
+        ref.free(hh)
+        vrng = np.random.default_rng(77)
+        size = np.array([130, 20, 300], dtype=np.uint32)
+        cells = vrng.choice(int(np.prod(size)), size=1500, replace=False).astype(np.uint32)
+        ref.vox_write(size, cells, 200, path)
+        data = np.frombuffer(path.read_bytes(), dtype=np.uint8)
+        np.savez_compressed(OUT / "vox_write_multi.npz", size=size, cells=cells, color_index=np.int64(200), vox_bytes=data)
+        summary["vox_write_multi"] = {"cells": int(cells.size), "bytes": int(data.size)}
+
     (OUT / "summary.json").write_text(json.dumps(summary, indent=1) + "\n")
     print(json.dumps(summary, indent=1))
     return 0
