@@ -1431,6 +1431,18 @@ int capture_batch(tg_batch* b)
 	while (pts > 2 && lds_for(pts, waves) > kMaxLds) pts >>= 1;
 	while (waves > 1 && lds_for(pts, waves) > kMaxLds) waves >>= 1;
 	ba.slots = block_spill_slots(ba.max_depth, pts);
+	{
+		// A persistent launch has a fixed number of workgroups per CU: the
+		// spill region may take the LDS they leave (deep tiles then keep all
+		// their points per lane).
+		const size_t per_wg = kMaxLds / size_t(std::max(1, b->workgroups_per_cu));
+		const size_t fixed = block_table_bytes(ba.max_binary, ba.max_len) + size_t(waves) * block_list_bytes(ba.max_len);
+		if (per_wg > fixed)
+		{
+			const uint32_t roomy = block_spill_slots(ba.max_depth, pts, (per_wg - fixed) / size_t(waves));
+			if (block_lds_bytes(roomy, pts, waves, ba.max_binary, ba.max_len) <= kMaxLds) ba.slots = std::max(ba.slots, roomy);
+		}
+	}
 	if (block_lds_bytes(ba.slots, pts, waves, ba.max_binary, ba.max_len) > kMaxLds)
 		return fail(TG_ERR_LIMIT, "batch: block programs too long for the LDS");
 	const uint32_t wgs = uint32_t(std::max(1, b->cus * b->workgroups_per_cu));

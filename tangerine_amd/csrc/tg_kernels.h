@@ -296,13 +296,14 @@ hipError_t launch_gradient(const GradArgs& a, hipStream_t s);
 // program needs more than `slots` slots is interpreted one point per lane,
 // seeing the region as slots * P one-float slots, which must hold the deepest
 // block program (the wave-0 decision pass runs one point per lane too).
-// Beyond that minimum the region aims at about 3 KiB per wave: enough for
-// the short programs of most tiles, small enough for high occupancy.
-__host__ __device__ inline uint32_t block_spill_slots(uint32_t max_depth, int p)
+// Beyond that minimum the region takes up to `budget_bytes` per wave (by
+// default 3 KiB: enough for the short programs of most tiles, small enough
+// for high occupancy).
+__host__ __device__ inline uint32_t block_spill_slots(uint32_t max_depth, int p, size_t budget_bytes = 3072)
 {
 	const uint32_t need = max_depth > 1 ? max_depth - 1 : 1;  // one-float slots
 	const uint32_t floor_slots = (need + uint32_t(p) - 1) / uint32_t(p);
-	const uint32_t budget = 3072u / (uint32_t(p) * 64u * 4u);
+	const uint32_t budget = uint32_t(budget_bytes / (size_t(p) * 64u * 4u));
 	const uint32_t want = budget < need ? budget : need;
 	return floor_slots > want ? floor_slots : (want > 0 ? want : 1u);
 }
