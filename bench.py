@@ -41,6 +41,14 @@ PEAK_FP32_TFLOPS = 157.3          # MI355X published FP32 vector peak (packed is
 PEAK_FP32_NONPACKED_TFLOPS = 78.6  # same without packed FP32 (not used by these kernels)
 PEAK_HBM_GBS = 8000.0
 
+# HBM bytes per k_eval_blocks launch, measured with rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE (separate runs) on the plain exact export at 1024^3 (launch
+# 2 points per lane, 4 waves), each counter scaled by what it reads for a
+# 1 GiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).  The algorithmic
+# bytes of that launch are 4.54e9 (2,146,689 tiles x 2048 B of samples plus
+# 64 B of sign bits each); counters cannot be read inside a timed run.
+MEASURED_TRAFFIC_BYTES = {"export1024": 4.637e9}
+
 WORKLOADS = {
     "export1024": dict(seed=1, primitives=2048, lo=-8.0, hi=8.0, cells=1024),
     "export512": dict(seed=1, primitives=256, lo=-6.0, hi=6.0, cells=512),
@@ -344,7 +352,11 @@ def main(argv=None) -> int:
         "batch64": batch,
         "roofline": {
             "bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": MEASURED_TRAFFIC_BYTES.get(args.workload) if prune and not args.fast else None,
+            "traffic_gbs": (MEASURED_TRAFFIC_BYTES[args.workload] / (launch_ms * 1e-3) / 1e9
+                            if prune and not args.fast and args.workload in MEASURED_TRAFFIC_BYTES else None),
+            "hbm_peak_gbs": PEAK_HBM_GBS,
             "frac_of_nonpacked_peak": achieved / PEAK_FP32_NONPACKED_TFLOPS,
             "kernel": "k_eval_blocks" if prune else "k_eval_tiles", "flops_per_launch": flops, "launch_ms": launch_ms,
             "launch": {"points_per_lane": st["points_per_lane"], "waves": st["waves"],
