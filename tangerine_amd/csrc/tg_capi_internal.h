@@ -41,6 +41,7 @@ int upload_program(const DecodedProgram& d, tg_program** out);
 struct CellPrograms
 {
 	DevBuf pool, desc, counters, scratch;
+	DevBuf values, levels;    // level-parallel decision pass: entry values per cell, entries by height
 	uint64_t ncells = 0;
 	uint64_t entries = 0;     // pool entries in use
 	uint32_t max_stack = 0;   // over all cells
@@ -54,6 +55,8 @@ struct CellPrograms
 		desc.release();
 		counters.release();
 		scratch.release();
+		values.release();
+		levels.release();
 	}
 };
 

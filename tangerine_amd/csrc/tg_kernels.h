@@ -90,6 +90,29 @@ struct PruneArgs
 	unsigned long long* counters;
 	uint64_t* scratch;      // per wave: 2 n_binary + n_entries words (decision masks, liveness)
 	uint32_t cells_per_wave;  // 1..64 (lanes beyond carry no cell)
+	int masks_ready;          // decision masks already in scratch (k_prune_level): skip the decision pass
+};
+
+// Level-parallel decision pass: one launch per entry height (leaves 0, a
+// parent one above its highest child); wave = (entry of the level, cell
+// group of k_prune_cells), lane = cell.  Every entry's value at every cell
+// centre is kept in `values`, so a parent reads its operands instead of
+// re-running the program; binary entries record the same decision masks as
+// k_prune_cells' own pass.
+struct LevelArgs
+{
+	const Instr* prog;
+	const Entry* entries;
+	const uint32_t* list;     // entry positions of this height
+	uint32_t count;           // entries of this height
+	uint32_t ngroups;         // cell groups (waves of k_prune_cells)
+	uint32_t groups_per_wave; // consecutive cell groups one wave runs (the entry's record loaded once)
+	uint32_t cells_per_wave;
+	float ox, oy, oz, cx, cy, cz, radius;
+	uint32_t nx, ny, nz;
+	float* values;            // [entry][group][64 lanes]
+	uint64_t* scratch;        // decision masks, as PruneArgs::scratch
+	uint32_t n_binary, n_entries;
 };
 
 // Level 2 + evaluation: one workgroup per block.
@@ -262,6 +285,7 @@ struct GradArgs
 // fast = contracted FP32).  `p` is points per lane for the tile kernel (4 or 8).
 namespace exact {
 hipError_t launch_points(const PointArgs& a, hipStream_t s);
+hipError_t launch_prune_level(const LevelArgs& a, hipStream_t s);
 hipError_t launch_vox(const VoxArgs& a, int stage, hipStream_t s);  // 0 select, 1 list
 hipError_t launch_pivot(const PivotArgs& a, hipStream_t s);
 hipError_t launch_cloud(const CloudArgs& a, int stage, hipStream_t s);  // 0 select, 1 points, 2 refine
@@ -276,6 +300,7 @@ hipError_t launch_gradient(const GradArgs& a, hipStream_t s);
 }
 namespace fast {
 hipError_t launch_points(const PointArgs& a, hipStream_t s);
+hipError_t launch_prune_level(const LevelArgs& a, hipStream_t s);
 hipError_t launch_vox(const VoxArgs& a, int stage, hipStream_t s);  // 0 select, 1 list
 hipError_t launch_pivot(const PivotArgs& a, hipStream_t s);
 hipError_t launch_cloud(const CloudArgs& a, int stage, hipStream_t s);  // 0 select, 1 points, 2 refine

@@ -3,12 +3,50 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "../../include/tangerine_gpu.h"
 #include "tg_capi_internal.h"
 #include "tg_kernels.h"
 
 namespace tg {
+
+namespace {
+
+constexpr uint32_t kMaxLevels = 64;                 // more heights: the in-kernel decision pass
+constexpr uint64_t kMaxValueBytes = 2ull << 30;    // entry values kept for the level-parallel pass
+
+// Entry positions ordered by height (leaves 0, a parent one above its highest
+// child) and the first position of every height.  Returns false when the
+// program has more than kMaxLevels heights.
+bool entry_levels(const DecodedProgram& d, std::vector<uint32_t>& order, std::vector<uint32_t>& first)
+{
+	const size_t n = d.entries.size();
+	std::vector<uint32_t> h(n, 0);
+	uint32_t top = 0;
+	for (size_t e = 0; e < n; ++e)
+	{
+		const HostEntry& x = d.entries[e];
+		const uint32_t op = x.rec >> 24;
+		if (op - OP_SPHERE <= OP_PLANE - OP_SPHERE)
+			h[e] = 0;
+		else if (op == OP_FLATE || op == OP_SCALE_FIELD)
+			h[e] = h[x.l] + 1;
+		else
+			h[e] = std::max(h[x.l], h[x.r]) + 1;
+		top = std::max(top, h[e]);
+		if (top >= kMaxLevels) return false;
+	}
+	first.assign(top + 2, 0);
+	for (size_t e = 0; e < n; ++e) ++first[h[e] + 1];
+	for (uint32_t k = 1; k < first.size(); ++k) first[k] += first[k - 1];
+	order.assign(n, 0);
+	std::vector<uint32_t> at(first.begin(), first.end() - 1);
+	for (size_t e = 0; e < n; ++e) order[at[h[e]]++] = uint32_t(e);
+	return true;
+}
+
+}  // namespace
 
 int prune_cells(hipStream_t stream, const tg_program* p, const float origin[3], const float cell[3], float radius,
 	const uint32_t counts[3], bool fast, CellPrograms& out)
@@ -29,6 +67,20 @@ int prune_cells(hipStream_t stream, const tg_program* p, const float origin[3], 
 	if (!rc) rc = out.counters.ensure(64);
 	if (!rc) rc = out.scratch.ensure(waves * prune_scratch_words(d.binary_entries, uint32_t(d.entries.size())) * 8);
 	if (rc) return rc;
+	// Level-parallel decision pass when the heights are few and the entry
+	// values fit: it replaces the per-wave walk over the whole program.
+	std::vector<uint32_t> level_order, level_first;
+	bool level_parallel = entry_levels(d, level_order, level_first) &&
+	                      uint64_t(d.entries.size()) * waves * 64 * 4 <= kMaxValueBytes;
+	if (level_parallel)
+	{
+		rc = out.values.ensure(uint64_t(d.entries.size()) * waves * 64 * 4);
+		if (!rc) rc = out.levels.ensure(level_order.size() * 4);
+		if (rc) return rc;
+		hipError_t e = hipMemcpyAsync(out.levels.ptr, level_order.data(), level_order.size() * 4,
+			hipMemcpyHostToDevice, stream);
+		if (e != hipSuccess) return hip_fail(e, "level upload");
+	}
 	// First guess: a quarter of the whole program per cell, at most 1 GiB; grown
 	// to the exact total and rerun if any cell did not fit.
 	uint64_t capacity = std::max<uint64_t>(out.pool.bytes / sizeof(Entry),
@@ -56,6 +108,35 @@ int prune_cells(hipStream_t stream, const tg_program* p, const float origin[3], 
 		pa.counters = out.counters.as<unsigned long long>();
 		pa.scratch = out.scratch.as<uint64_t>();
 		pa.cells_per_wave = cpw;
+		if (attempt == 0 && level_parallel)
+		{
+			LevelArgs la{};
+			la.prog = pa.prog;
+			la.entries = pa.entries;
+			la.ngroups = uint32_t(waves);
+			la.cells_per_wave = cpw;
+			la.ox = pa.ox; la.oy = pa.oy; la.oz = pa.oz;
+			la.cx = pa.cx; la.cy = pa.cy; la.cz = pa.cz;
+			la.radius = radius;
+			la.nx = pa.nx; la.ny = pa.ny; la.nz = pa.nz;
+			la.values = out.values.as<float>();
+			la.scratch = pa.scratch;
+			la.n_binary = pa.n_binary;
+			la.n_entries = pa.n_entries;
+			for (size_t h = 0; h + 1 < level_first.size(); ++h)
+			{
+				la.list = out.levels.as<const uint32_t>() + level_first[h];
+				la.count = level_first[h + 1] - level_first[h];
+				// a few cell groups per wave while enough waves remain to fill the GPU
+				la.groups_per_wave = 1;
+				while (la.groups_per_wave < 8 && uint64_t(la.count) * waves / (la.groups_per_wave * 2) >= 8192)
+					la.groups_per_wave *= 2;
+				e = fast ? fast::launch_prune_level(la, stream) : exact::launch_prune_level(la, stream);
+				if (e != hipSuccess) return hip_fail(e, "level pruning launch");
+			}
+		}
+		// the masks stay in scratch for the rerun after a pool overflow
+		pa.masks_ready = level_parallel ? 1 : 0;
 		e = fast ? fast::launch_prune_cells(pa, stream) : exact::launch_prune_cells(pa, stream);
 		if (e != hipSuccess) return hip_fail(e, "cell pruning launch");
 		unsigned long long c[5] = { 0, 0, 0, 0, 0 };
