@@ -41,7 +41,7 @@ int upload_program(const DecodedProgram& d, tg_program** out);
 struct CellPrograms
 {
 	DevBuf pool, desc, counters, scratch;
-	DevBuf values, levels;    // level-parallel decision pass: entry values per cell, entries by height
+	DevBuf values, levels, lens;  // level-parallel passes: entry values per cell, entries by height, program lengths
 	uint64_t ncells = 0;
 	uint64_t entries = 0;     // pool entries in use
 	uint32_t max_stack = 0;   // over all cells
@@ -57,6 +57,7 @@ struct CellPrograms
 		scratch.release();
 		values.release();
 		levels.release();
+		lens.release();
 	}
 };
 

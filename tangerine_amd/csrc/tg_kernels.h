@@ -90,7 +90,8 @@ struct PruneArgs
 	unsigned long long* counters;
 	uint64_t* scratch;      // per wave: 2 n_binary + n_entries words (decision masks, liveness)
 	uint32_t cells_per_wave;  // 1..64 (lanes beyond carry no cell)
-	int masks_ready;          // decision masks already in scratch (k_prune_level): skip the decision pass
+	int masks_ready;          // decision masks and liveness already in scratch (k_prune_level): skip both passes
+	const uint32_t* counts;   // with masks_ready: program length per lane, [group][64]
 };
 
 // Level-parallel decision pass: one launch per entry height (leaves 0, a
@@ -111,8 +112,10 @@ struct LevelArgs
 	float ox, oy, oz, cx, cy, cz, radius;
 	uint32_t nx, ny, nz;
 	float* values;            // [entry][group][64 lanes]
-	uint64_t* scratch;        // decision masks, as PruneArgs::scratch
+	uint64_t* scratch;        // decision masks and liveness, as PruneArgs::scratch
 	uint32_t n_binary, n_entries;
+	int liveness;             // 0: values and decisions (heights upward); 1: liveness (heights downward)
+	uint32_t* counts;         // liveness: entries each lane's program will hold, [group][64] (zeroed)
 };
 
 // Level 2 + evaluation: one workgroup per block.

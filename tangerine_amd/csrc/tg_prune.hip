@@ -76,6 +76,7 @@ int prune_cells(hipStream_t stream, const tg_program* p, const float origin[3], 
 	{
 		rc = out.values.ensure(uint64_t(d.entries.size()) * waves * 64 * 4);
 		if (!rc) rc = out.levels.ensure(level_order.size() * 4);
+		if (!rc) rc = out.lens.ensure(waves * 64 * 4);
 		if (rc) return rc;
 		hipError_t e = hipMemcpyAsync(out.levels.ptr, level_order.data(), level_order.size() * 4,
 			hipMemcpyHostToDevice, stream);
@@ -123,8 +124,15 @@ int prune_cells(hipStream_t stream, const tg_program* p, const float origin[3], 
 			la.scratch = pa.scratch;
 			la.n_binary = pa.n_binary;
 			la.n_entries = pa.n_entries;
-			for (size_t h = 0; h + 1 < level_first.size(); ++h)
+			la.counts = out.lens.as<uint32_t>();
+			e = hipMemsetAsync(out.lens.ptr, 0, waves * 64 * 4, stream);
+			if (e != hipSuccess) return hip_fail(e, "length reset");
+			const size_t heights = level_first.size() - 1;
+			// values and decisions upward, then liveness and lengths downward
+			for (size_t step = 0; step < 2 * heights; ++step)
 			{
+				const size_t h = step < heights ? step : 2 * heights - 1 - step;
+				la.liveness = step < heights ? 0 : 1;
 				la.list = out.levels.as<const uint32_t>() + level_first[h];
 				la.count = level_first[h + 1] - level_first[h];
 				// a few cell groups per wave while enough waves remain to fill the GPU
@@ -135,8 +143,9 @@ int prune_cells(hipStream_t stream, const tg_program* p, const float origin[3], 
 				if (e != hipSuccess) return hip_fail(e, "level pruning launch");
 			}
 		}
-		// the masks stay in scratch for the rerun after a pool overflow
+		// masks, liveness and lengths stay in scratch for the rerun after a pool overflow
 		pa.masks_ready = level_parallel ? 1 : 0;
+		pa.counts = out.lens.as<const uint32_t>();
 		e = fast ? fast::launch_prune_cells(pa, stream) : exact::launch_prune_cells(pa, stream);
 		if (e != hipSuccess) return hip_fail(e, "cell pruning launch");
 		unsigned long long c[5] = { 0, 0, 0, 0, 0 };
