@@ -102,6 +102,30 @@ def test_deep_stack_and_partial_blocks(mesher, oracle):
     oracle.free(h)
 
 
+@pytest.mark.parametrize("length", [40, 90])
+def test_long_chains(mesher, oracle, length):
+    """A left-deep chain of `length` blended spheres: its entry heights (40
+    and 90) fall on both sides of the 64 heights the level-parallel decision
+    pass takes (beyond that the pruning kernel walks the program itself);
+    both give the oracle's grid bit for bit."""
+    rng = np.random.default_rng(length)
+    chain = [("sphere", 0.6)]
+    for i in range(length - 1):
+        x, y, z = (float(v) for v in rng.uniform(-2.0, 2.0, 3))
+        chain += [("sphere", float(rng.uniform(0.3, 0.9))), ("move", x, y, z),
+                  ("blend_union", 0.2) if i % 3 else ("union",)]
+    h = oracle.build(chain)
+    words = oracle.compile(h)
+    from tangerine_amd.sdf import Program
+    prog = Program.from_words(words)
+    origin, step, cells = (-3.0, -3.0, -3.0), (0.05,) * 3, (120, 120, 120)
+    st = mesher.export(prog, origin, step, cells)
+    expect, _ = oracle.grid_interp(words, origin, step, cells, threads=8)
+    assert np.array_equal(bits(mesher.grid()), bits(expect.ravel()))
+    assert st["exact_evals"] > 0
+    oracle.free(h)
+
+
 def test_every_tuning_gives_the_same_grid(mesher):
     """Every (points per lane, waves, shortest pruned block program) setting
     of the block kernel writes the same bits, and the autotuner's choice is
