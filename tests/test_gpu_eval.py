@@ -101,6 +101,27 @@ def test_point_edge_cases(gpu):
     np.testing.assert_allclose(d, np.linalg.norm(pts, axis=1) - 1.0, atol=1e-6)
 
 
+def test_sqrt_special_arguments_bit_identical(gpu, oracle):
+    """Square-root arguments at the edges of the range: points at a sphere's
+    centre (argument 0), at tiny offsets (arguments 1e-40 .. 1e-30, where
+    the correctly rounded sequence scales its input) and far away (1e36,
+    near overflow), alone and mixed into waves of ordinary points, give the
+    oracle's bits (glibc's correctly rounded sqrtf)."""
+    from tangerine_amd import sdf
+    node = sdf.sphere(2)
+    prog, words = node.program(), node.compile()
+    rng = np.random.default_rng(5)
+    special = np.float32([[0, 0, 0], [1e-20, 0, 0], [0, 3e-19, 0], [1e-15, 1e-15, 1e-15], [2e-24, 0, 7e-25],
+                          [1e18, 0, 0], [1e19, 1e19, 0], [-0.0, 0, 0], [5e-20, -5e-20, 0]])
+    ordinary = rng.uniform(-3, 3, size=(4096, 3)).astype(np.float32)
+    mixed = ordinary.copy()
+    mixed[::97] = special[np.arange(len(mixed[::97])) % len(special)]
+    for pts in (special, ordinary, mixed, np.repeat(special, 64, axis=0)):
+        d = prog.eval_points(pts)
+        ref = oracle.interp_words(words, pts)
+        assert np.array_equal(d.view(np.uint32), ref.view(np.uint32))
+
+
 # ---- grid export ---------------------------------------------------------------
 
 class Mesher:
