@@ -335,9 +335,14 @@ __host__ __device__ inline uint32_t block_spill_slots(uint32_t max_depth, int p,
 	const uint32_t want = budget < need ? budget : need;
 	return floor_slots > want ? floor_slots : (want > 0 ? want : 1u);
 }
+// Block kernel LDS: decision masks and liveness of the block program, one
+// uint4 per tile (its program's place in the tile pool, length, stack depth,
+// cost), then every wave's program list (together: the tile pool), then
+// every wave's spill region.
+constexpr size_t kTileInfoBytes = 64 * 16;
 __host__ __device__ inline size_t block_table_bytes(uint32_t max_binary, uint32_t max_len)
 {
-	return size_t(max_binary) * 16 + ((size_t(max_len) * 8 + 15) & ~size_t(15));
+	return size_t(max_binary) * 16 + ((size_t(max_len) * 8 + 15) & ~size_t(15)) + kTileInfoBytes;
 }
 __host__ __device__ inline size_t block_list_bytes(uint32_t max_len)
 {
