@@ -379,6 +379,8 @@ typedef struct tg_batch_stats
 	uint64_t exact_evals;
 	double flops;
 	uint32_t points_per_lane, waves, workgroups;
+	uint32_t lds_bytes;        /* LDS per workgroup of the short-program launch (the long one if none) */
+	uint32_t max_len, max_binary, max_depth;  /* its layout: the largest over its queue */
 } tg_batch_stats;
 typedef struct tg_batch_tune_result
 {
@@ -395,7 +397,9 @@ int tg_batch_export(tg_batch* batch, const tg_program* const* programs, const tg
 /* Replays the captured evaluation launch of the last export (same trees). */
 int tg_batch_replay(tg_batch* batch, double* eval_ms);
 /* Times points per lane {2,4,8} x waves {2,4} x persistent workgroups per CU
- * {1,2,4} on the last exported batch and keeps the fastest. */
+ * {1,2,4} for the short-program queue (blocks of at most 256 entries) on the
+ * last exported batch, keeps the fastest, then picks the long-program queue's
+ * shape among four; eval_ms is the best whole-batch time seen. */
 int tg_batch_autotune(tg_batch* batch, tg_batch_tune_result* out);
 int tg_batch_set_tuning(tg_batch* batch, uint32_t points_per_lane, uint32_t waves, uint32_t workgroups_per_cu);
 tg_mesher* tg_batch_mesher(tg_batch* batch, uint32_t index);

@@ -61,7 +61,8 @@ class TgTuneResult(C.Structure):
 class TgBatchStats(C.Structure):
     _fields_ = [("prune_ms", C.c_double), ("eval_ms", C.c_double), ("mesh_ms", C.c_double), ("total_ms", C.c_double),
                 ("trees", _u64), ("blocks", _u64), ("samples", _u64), ("exact_evals", _u64), ("flops", C.c_double),
-                ("points_per_lane", _u32), ("waves", _u32), ("workgroups", _u32)]
+                ("points_per_lane", _u32), ("waves", _u32), ("workgroups", _u32), ("lds_bytes", _u32),
+                ("max_len", _u32), ("max_binary", _u32), ("max_depth", _u32)]
 
 
 class TgBatchTuneResult(C.Structure):
