@@ -149,6 +149,11 @@ int tg_device_synchronize(void);
  * device allocations it makes itself with a plain streaming kernel, `repeat`
  * times; used to calibrate HBM byte counters against a known traffic. */
 int tg_device_copy_probe(uint64_t bytes, int repeat, double* ms_per_copy);
+/* Self-check (no reference counterpart): evaluates the exact build's square
+ * root for all 2^32 float inputs against the compiler's correctly rounded
+ * sqrtf and counts results that differ (NaNs compare equal).  scramble = 0
+ * gives each wave 64 consecutive bit patterns, 1 a bijective mix of them. */
+int tg_check_sqrt(uint32_t scramble, uint64_t* mismatches);
 
 /* ---- tree builders (SDF::* factories; radii and half extents) -------------- */
 tg_node* tg_sphere(float radius);                                        /* sdf_evaluator.cpp:1206 */

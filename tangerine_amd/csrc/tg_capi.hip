@@ -393,6 +393,21 @@ extern "C" int tg_eval_points_device(const tg_program* p, const float* xyz, uint
 	return TG_OK;
 }
 
+extern "C" int tg_check_sqrt(uint32_t scramble, uint64_t* mismatches)
+{
+	if (!mismatches) return fail(TG_ERR_INVALID, "tg_check_sqrt: null argument");
+	DevBuf d;
+	int rc = d.ensure(8);
+	if (rc) return rc;
+	struct Guard { DevBuf& a; ~Guard() { a.release(); } } g{ d };
+	TG_CHECK_HIP(hipMemset(d.ptr, 0, 8), "reset");
+	TG_CHECK_HIP(exact::launch_sqrt_check(scramble, d.as<unsigned long long>(), nullptr), "sqrt check");
+	unsigned long long n = 0;
+	TG_CHECK_HIP(hipMemcpy(&n, d.ptr, 8, hipMemcpyDeviceToHost), "sqrt check");
+	*mismatches = n;
+	return TG_OK;
+}
+
 extern "C" int tg_eval_points(const tg_program* p, const float* xyz, uint64_t n, float* out, uint32_t flags)
 {
 	if (!p || (n && (!xyz || !out))) return fail(TG_ERR_INVALID, "tg_eval_points: null argument");

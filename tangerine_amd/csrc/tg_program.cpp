@@ -179,6 +179,15 @@ bool decode_words(const uint32_t* w, size_t n, DecodedProgram& out, std::string&
 				const float t = as_float(w[pc++]);
 				s.w[3] = as_word(t);
 				s.w[4] = as_word(0.25f / t);
+				// exact build: RN(0.25 / t) and 4 t in double, low word first
+				const double yq = 0.25 / double(t), t4 = 4.0 * double(t);
+				uint64_t b[2];
+				std::memcpy(&b[0], &yq, 8);
+				std::memcpy(&b[1], &t4, 8);
+				s.w[5] = uint32_t(b[0]);
+				s.w[6] = uint32_t(b[0] >> 32);
+				s.w[7] = uint32_t(b[1]);
+				s.w[8] = uint32_t(b[1] >> 32);
 			}
 			out.instrs.push_back(s);
 			out.set_ops++;

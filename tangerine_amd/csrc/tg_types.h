@@ -47,7 +47,9 @@ enum XformKind : uint32_t
 // One instruction = 16 dwords.
 //   brush:  w[0] = opcode | kind << 8;  w[1..12] = 3x4 matrix rows; w[13..15] = params
 //   set op: w[0] = opcode;  w[1] = lhs subtree start; w[2] = rhs subtree start;
-//           f[3] = threshold; f[4] = 0.25 / threshold (blend only)
+//           f[3] = threshold; f[4] = 0.25 / threshold (blend only);
+//           w[5..6] = RN(0.25 / threshold), w[7..8] = 4 threshold as doubles,
+//           low word first (blend only)
 //   flate / scale field: f[1] = radius / scale
 //   negate / stop: w[0] only
 struct Instr

@@ -122,6 +122,20 @@ def test_sqrt_special_arguments_bit_identical(gpu, oracle):
         assert np.array_equal(d.view(np.uint32), ref.view(np.uint32))
 
 
+@pytest.mark.parametrize("scramble", [0, 1])
+def test_sqrt_all_inputs_match_sqrtf(gpu, scramble):
+    """All 2^32 float inputs: the exact build's square root (hardware root,
+    residual selection, sqrtf itself for a wave holding 0 < x < 2^-96) gives
+    the bits of the compiler's correctly rounded sqrtf, with waves of
+    consecutive bit patterns and with every wave mixing tiny, zero, negative,
+    infinite, NaN and ordinary inputs."""
+    from tangerine_amd import _lib
+    lib = _lib.load()
+    n = C.c_uint64(1)
+    _lib.check(lib.tg_check_sqrt(scramble, C.byref(n)), "tg_check_sqrt")
+    assert n.value == 0
+
+
 # ---- grid export ---------------------------------------------------------------
 
 class Mesher:
