@@ -46,13 +46,13 @@ struct GridArgs
 // `rec`" (or negate the top of stack), and where its operands' roots are.
 struct Entry
 {
-	uint32_t rec;   // record index (bits 0..23) | opcode << 24 (OP_NEGATE: no record)
+	uint32_t rec;   // record index (bits 0..21) | transform kind << 22 | opcode << 24 (OP_NEGATE: no record)
 	uint32_t l;     // binary: position of the left operand's root; unary: operand root
 	uint32_t r;     // binary: position of the right operand's root
 	uint32_t ord;   // binary: ordinal among the program's binary entries
 };
 
-constexpr uint32_t kRecMask = 0x00ffffffu;
+constexpr uint32_t kRecMask = 0x003fffffu;
 constexpr int kBlockTiles = 4;                 // tiles per block edge
 constexpr int kTilesPerBlock = 64;             // 4^3 tiles = one lane each
 constexpr int kBlockSamples = kTile * kBlockTiles;

@@ -273,7 +273,7 @@ bool decode_words(const uint32_t* w, size_t n, DecodedProgram& out, std::string&
 		in.w[0] = (in.w[0] & 0xffffu) | (uint32_t(c) << 16);
 		const uint32_t op = instr_op(in);
 		if (op == OP_STOP) break;
-		HostEntry e{ uint32_t(i) | (op << 24), 0, 0, 0 };
+		HostEntry e{ uint32_t(i) | (instr_kind(in) << 22) | (op << 24), 0, 0, 0 };
 		if (is_binary(op))
 		{
 			e.l = in.w[2] - 1;   // the left operand ends right before the right one starts
@@ -287,9 +287,9 @@ bool decode_words(const uint32_t* w, size_t n, DecodedProgram& out, std::string&
 		out.entries.push_back(e);
 	}
 	out.binary_entries = ord;
-	if (out.instrs.size() >= (1u << 24))
+	if (out.instrs.size() >= (1u << 22))
 	{
-		err = "program exceeds 2^24 records";
+		err = "program exceeds 2^22 records";
 		return false;
 	}
 	return true;

@@ -9,7 +9,7 @@
 
 namespace tg {
 
-// Entry view of a program (see Entry in tg_kernels.h): rec | op << 24, l, r, ord.
+// Entry view of a program (see Entry in tg_kernels.h): rec | kind << 22 | op << 24, l, r, ord.
 struct HostEntry
 {
 	uint32_t rec, l, r, ord;

@@ -122,6 +122,27 @@ def test_sqrt_special_arguments_bit_identical(gpu, oracle):
         assert np.array_equal(d.view(np.uint32), ref.view(np.uint32))
 
 
+@pytest.mark.parametrize("name", ["points_config1.npz", "points_rand16_s5.npz", "points_rand256_s1.npz"])
+def test_non_finite_points_match_oracle(gpu, oracle, name):
+    """Infinite and NaN coordinates (where rays that march off to infinity
+    end up): the device gives the oracle's value wherever that is a number and
+    NaN wherever it is NaN, so no such point passes a d <= eps test that the
+    reference's would fail."""
+    from tangerine_amd.sdf import Program
+    g = load_golden(name)
+    prog = Program.from_words(g["words"])
+    inf, nan = np.inf, np.nan
+    pts = np.float32([[inf, 0, 0], [-inf, 1, 2], [0, inf, 0], [0, 0, -inf], [inf, inf, inf], [inf, -inf, 0],
+                      [nan, 0, 0], [0, nan, 1], [1, 2, nan], [nan, nan, nan], [inf, nan, 0], [1e30, 1e30, 1e30],
+                      [3e38, 0, 0], [-3e38, 3e38, 1]])
+    pts = np.concatenate([pts, np.random.default_rng(1).uniform(-6, 6, size=(50, 3)).astype(np.float32)])
+    d = prog.eval_points(pts)
+    ref = oracle.interp_words(g["words"], pts)
+    assert np.array_equal(np.isnan(d), np.isnan(ref))
+    ok = ~np.isnan(ref)
+    assert np.array_equal(d[ok].view(np.uint32), ref[ok].view(np.uint32))
+
+
 @pytest.mark.parametrize("scramble", [0, 1])
 def test_sqrt_all_inputs_match_sqrtf(gpu, scramble):
     """All 2^32 float inputs: the exact build's square root (hardware root,
