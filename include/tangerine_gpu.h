@@ -404,7 +404,7 @@ int tg_batch_replay(tg_batch* batch, double* eval_ms);
 /* Times points per lane {2,4,8} x waves {2,4} x persistent workgroups per CU
  * {1,2,4} for the short-program queue (blocks of at most 256 entries) on the
  * last exported batch, keeps the fastest, then picks the long-program queue's
- * shape among four; eval_ms is the best whole-batch time seen. */
+ * shape among seven; eval_ms is the best whole-batch time seen. */
 int tg_batch_autotune(tg_batch* batch, tg_batch_tune_result* out);
 int tg_batch_set_tuning(tg_batch* batch, uint32_t points_per_lane, uint32_t waves, uint32_t workgroups_per_cu);
 tg_mesher* tg_batch_mesher(tg_batch* batch, uint32_t index);

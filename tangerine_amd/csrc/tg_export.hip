@@ -165,7 +165,10 @@ int surface_nets(tg_mesher* m)
 	return TG_OK;
 }
 
-constexpr size_t kMaxLds = 160 * 1024;
+// Dynamic LDS a workgroup may ask for: the CU's 160 KiB less room for the
+// kernels' static LDS (the persistent batch kernel's queue item), so that
+// the persistent launches that split the CU's LDS evenly never overshoot.
+constexpr size_t kMaxLds = 160 * 1024 - 256;
 
 // Thread per cube brick: a kept brick (cubes [8b, 8b + 8) per axis, corner
 // samples [8b, 8b + 8]) needs sample tiles b and b + 1 on each axis.  Sets
@@ -1473,19 +1476,29 @@ int shape_queue(tg_batch* b, BatchArgs& ba, int points, int waves_in, int wgs_pe
 	while (pts > 2 && lds_for(pts, waves) > kMaxLds) pts >>= 1;
 	while (waves > 1 && lds_for(pts, waves) > kMaxLds) waves >>= 1;
 	ba.slots = block_spill_slots(ba.max_depth, pts);
+	ba.pool_words = 0;
 	{
 		// A persistent launch has a fixed number of workgroups per CU: the
-		// spill region may take the LDS they leave (deep tiles then keep all
-		// their points per lane).
+		// LDS they leave goes first to the spill region (deep tiles then keep
+		// all their points per lane), then to the tile pool, up to what the
+		// 64 tile programs of the longest block can need (a wave emitting its
+		// own tile programs, one walk of the block program per tile, is the
+		// slow path when they do not fit).
 		const size_t per_wg = kMaxLds / size_t(std::max(1, wgs_per_cu));
-		const size_t fixed = block_table_bytes(ba.max_binary, ba.max_len) + size_t(waves) * block_list_bytes(ba.max_len);
+		const size_t lists = size_t(waves) * block_list_bytes(ba.max_len);
+		const size_t fixed = block_table_bytes(ba.max_binary, ba.max_len) + lists;
 		if (per_wg > fixed)
 		{
 			const uint32_t roomy = block_spill_slots(ba.max_depth, pts, (per_wg - fixed) / size_t(waves));
-			if (block_lds_bytes(roomy, pts, waves, ba.max_binary, ba.max_len) <= kMaxLds) ba.slots = std::max(ba.slots, roomy);
+			if (block_lds_bytes(roomy, pts, waves, ba.max_binary, ba.max_len) <= per_wg) ba.slots = std::max(ba.slots, roomy);
+			const size_t used = block_lds_bytes(ba.slots, pts, waves, ba.max_binary, ba.max_len);
+			const size_t want = size_t(kTilesPerBlock) * block_list_bytes(ba.max_len);
+			const size_t extra = per_wg > used ? std::min(want > lists ? want - lists : size_t(0),
+			                                              (per_wg - used) & ~size_t(15)) : size_t(0);
+			ba.pool_words = uint32_t((lists + extra) / 4);
 		}
 	}
-	if (block_lds_bytes(ba.slots, pts, waves, ba.max_binary, ba.max_len) > kMaxLds)
+	if (block_lds_bytes(ba.slots, pts, waves, ba.max_binary, ba.max_len, ba.pool_words) > kMaxLds)
 		return fail(TG_ERR_LIMIT, "batch: block programs too long for the LDS");
 	out.pts = pts;
 	out.waves = waves;
@@ -1779,7 +1792,7 @@ extern "C" int tg_batch_export(tg_batch* b, const tg_program* const* programs, c
 	st.workgroups = uint32_t(b->last_workgroups);
 	const BatchArgs& q = b->args[b->args[0].total ? 0 : 1];
 	st.lds_bytes = uint32_t(block_lds_bytes(q.slots, std::max(2, b->last_points_per_lane), std::max(1, b->last_waves),
-		q.max_binary, q.max_len));
+		q.max_binary, q.max_len, q.pool_words));
 	st.max_len = q.max_len;
 	st.max_binary = q.max_binary;
 	st.max_depth = q.max_depth;
@@ -1845,10 +1858,11 @@ extern "C" int tg_batch_autotune(tg_batch* b, tg_batch_tune_result* out)
 	if (b->args[1].total)
 	{
 		// the long-program queue: a few shapes, the short queue's choice fixed
-		const int long_c[4][3] = { { 8, 4, 1 }, { 4, 4, 1 }, { 8, 2, 1 }, { 4, 2, 2 } };
+		const int long_c[7][3] = { { 8, 4, 1 }, { 4, 4, 1 }, { 8, 2, 1 }, { 4, 2, 2 }, { 2, 4, 2 }, { 4, 4, 2 },
+		                           { 2, 4, 1 } };
 		double best_long = 1e300;
 		int pick = 0;
-		for (int c = 0; c < 4; ++c)
+		for (int c = 0; c < 7; ++c)
 		{
 			b->long_points = long_c[c][0];
 			b->long_waves = long_c[c][1];

@@ -92,6 +92,7 @@ struct PruneArgs
 	uint32_t cells_per_wave;  // 1..64 (lanes beyond carry no cell)
 	int masks_ready;          // decision masks and liveness already in scratch (k_prune_level): skip both passes
 	const uint32_t* counts;   // with masks_ready: program length per lane, [group][64]
+	uint32_t splits;          // with masks_ready: waves per cell group (1, 2 or 4), each emitting 64 / splits columns
 };
 
 // Level-parallel decision pass: one launch per entry height (leaves 0, a
@@ -158,6 +159,7 @@ struct BatchArgs
 	unsigned int* head;          // queue head (reset to 0 before each launch)
 	uint32_t slots, max_binary, max_len;  // LDS layout: the largest over the trees
 	uint32_t max_depth;                   // deepest block program over the trees (host: sizes `slots`)
+	uint32_t pool_words;                  // tile pool (block_pool_words; 0: the waves' lists only)
 };
 
 // Exact brick certificate: wave per block (4^3 cube bricks of 8^3 cubes,
@@ -350,14 +352,19 @@ __host__ __device__ inline size_t block_list_bytes(uint32_t max_len)
 {
 	return (size_t(max_len) * 4 + 15) & ~size_t(15);
 }
-__host__ __device__ inline size_t block_wave_bytes(uint32_t slots, int p, uint32_t max_len)
+// Tile pool words: at least every wave's list (a wave emits a tile program
+// there when the block's 64 do not fit the pool together), more when the
+// launch asks for it; a multiple of 4 words.
+__host__ __device__ inline uint32_t block_pool_words(uint32_t pool_words, int waves, uint32_t max_len)
 {
-	return block_list_bytes(max_len) + spill_bytes_per_wave(slots, p);
+	const uint32_t lists = uint32_t(waves) * uint32_t(block_list_bytes(max_len) / 4);
+	return ((pool_words > lists ? pool_words : lists) + 3u) & ~3u;
 }
 __host__ __device__ inline size_t block_lds_bytes(uint32_t slots, int p, int waves, uint32_t max_binary,
-	uint32_t max_len)
+	uint32_t max_len, uint32_t pool_words = 0)
 {
-	return block_table_bytes(max_binary, max_len) + size_t(waves) * block_wave_bytes(slots, p, max_len);
+	return block_table_bytes(max_binary, max_len) + size_t(block_pool_words(pool_words, waves, max_len)) * 4 +
+	       size_t(waves) * spill_bytes_per_wave(slots, p);
 }
 // Cell pruning: per-lane position stacks and the spill region (the decision
 // masks and liveness live in global scratch, prune_scratch_words per wave).

@@ -146,6 +146,9 @@ int prune_cells(hipStream_t stream, const tg_program* p, const float origin[3], 
 		// masks, liveness and lengths stay in scratch for the rerun after a pool overflow
 		pa.masks_ready = level_parallel ? 1 : 0;
 		pa.counts = out.lens.as<const uint32_t>();
+		// emission only: four waves per cell group while that leaves under ~8
+		// waves per SIMD
+		pa.splits = (level_parallel && waves * 4 <= 8192) ? 4u : 1u;
 		e = fast ? fast::launch_prune_cells(pa, stream) : exact::launch_prune_cells(pa, stream);
 		if (e != hipSuccess) return hip_fail(e, "cell pruning launch");
 		unsigned long long c[5] = { 0, 0, 0, 0, 0 };
