@@ -170,31 +170,40 @@ int surface_nets(tg_mesher* m)
 // the persistent launches that split the CU's LDS evenly never overshoot.
 constexpr size_t kMaxLds = 160 * 1024 - 256;
 
-// Thread per cube brick: a kept brick (cubes [8b, 8b + 8) per axis, corner
-// samples [8b, 8b + 8]) needs sample tiles b and b + 1 on each axis.  Sets
-// their bits in the per-block tile masks (bit t = x + 4 y + 16 z inside a
-// block of 4^3 tiles) and counts kept bricks (counter[0]).
+// Wave per block of 4^3 sample tiles, lane = tile (bit t = x + 4 y + 16 z).
+// A kept cube brick b (cubes [8b, 8b + 8) per axis, corner samples
+// [8b, 8b + 8]) reads sample tiles b and b + 1 on each axis, so tile i is
+// needed when one of the bricks i - 1 and i (per axis) is kept: one ballot is
+// the block's mask, stored whole (no atomics).  The lane whose tile index is
+// a brick index counts that brick when it is kept (counter[0]).
 __global__ void __launch_bounds__(256) k_tiles_from_bricks(const uint8_t* flags, uint32_t cbx, uint32_t cby,
-	uint32_t cbz, uint32_t tx, uint32_t ty, uint32_t tz, uint32_t bx, uint32_t by, unsigned long long* mask,
-	unsigned long long* counter)
+	uint32_t cbz, uint32_t tx, uint32_t ty, uint32_t tz, uint32_t bx, uint32_t by, uint32_t bz,
+	unsigned long long* mask, unsigned long long* counter)
 {
-	const uint64_t b = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-	const uint64_t n = uint64_t(cbx) * cby * cbz;
-	const bool keep = b < n && flags[b] != 0;
-	const uint64_t votes = __ballot(keep);
-	if ((threadIdx.x & 63) == 0 && votes) atomicAdd(counter, (unsigned long long)__popcll(votes));
-	if (!keep) return;
-	const uint32_t x = uint32_t(b % cbx), y = uint32_t((b / cbx) % cby), z = uint32_t(b / (uint64_t(cbx) * cby));
-	for (uint32_t dz = 0; dz < 2; ++dz)
-		for (uint32_t dy = 0; dy < 2; ++dy)
-			for (uint32_t dx = 0; dx < 2; ++dx)
-			{
-				const uint32_t ix = x + dx, iy = y + dy, iz = z + dz;
-				if (ix >= tx || iy >= ty || iz >= tz) continue;
-				const uint32_t blk = (ix >> 2) + bx * ((iy >> 2) + by * (iz >> 2));
-				const uint32_t bit = (ix & 3) + 4 * (iy & 3) + 16 * (iz & 3);
-				atomicOr(&mask[blk], 1ull << bit);
-			}
+	const uint64_t w = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+	const uint32_t lane = threadIdx.x & 63;
+	if (w >= uint64_t(bx) * by * bz) return;  // whole waves
+	const uint32_t bxi = uint32_t(w % bx), byi = uint32_t((w / bx) % by), bzi = uint32_t(w / (uint64_t(bx) * by));
+	const uint32_t ix = bxi * 4 + (lane & 3), iy = byi * 4 + ((lane >> 2) & 3), iz = bzi * 4 + (lane >> 4);
+	bool need = false, own = false;
+	if (ix < tx && iy < ty && iz < tz)
+		for (uint32_t dz = 0; dz < 2; ++dz)
+			for (uint32_t dy = 0; dy < 2; ++dy)
+				for (uint32_t dx = 0; dx < 2; ++dx)
+				{
+					if (ix < dx || iy < dy || iz < dz) continue;
+					const uint32_t x = ix - dx, y = iy - dy, z = iz - dz;
+					if (x >= cbx || y >= cby || z >= cbz) continue;
+					const bool f = flags[x + uint64_t(cbx) * (y + uint64_t(cby) * z)] != 0;
+					need = need || f;
+					if (dx + dy + dz == 0) own = f;
+				}
+	const uint64_t m = __ballot(need), o = __ballot(own);
+	if (lane == 0)
+	{
+		mask[w] = m;
+		if (o) atomicAdd(counter, (unsigned long long)__popcll(o));
+	}
 }
 
 // Thread per block: list the blocks with at least one tile to evaluate.
@@ -446,8 +455,8 @@ int select_tiles(tg_mesher* m, const tg_program* p, const tg_grid* g, const tg_s
 		hipError_t e = (flags & TG_FLAG_FAST) ? fast::launch_certify(ca, m->stream) : exact::launch_certify(ca, m->stream);
 		if (e != hipSuccess) return hip_fail(e, "brick certificate launch");
 	}
-	hipLaunchKernelGGL(k_tiles_from_bricks, dim3(grid_for(nbricks)), dim3(256), 0, m->stream,
-		m->brick_flags.as<const uint8_t>(), ma.cbx, ma.cby, ma.cbz, ma.tx, ma.ty, ma.tz, bx, by,
+	hipLaunchKernelGGL(k_tiles_from_bricks, dim3(grid_for(uint64_t(nblocks) * 64)), dim3(256), 0, m->stream,
+		m->brick_flags.as<const uint8_t>(), ma.cbx, ma.cby, ma.cbz, ma.tx, ma.ty, ma.tz, bx, by, bz,
 		m->tile_mask.as<unsigned long long>(), ctr);
 	TG_CHECK_HIP(hipGetLastError(), "tile selection launch");
 	if (tiles)
