@@ -419,7 +419,10 @@ int tg_batch_stats_get(const tg_batch* batch, tg_batch_stats* out);
  * so it is skipped only when |prediction(c)| exceeds the largest |prediction|
  * seen at held-out calibration points with |exact d| <= r, plus the mean
  * calibration residual (or, after tg_surrogate_set_margin, when it exceeds
- * r + margin).  Every sample and vertex of the output still comes
+ * r + margin).  With TG_FLAG_CERTIFY every skipped brick is checked exactly,
+ * so the bar is the 90th percentile of those |prediction|s instead of the
+ * largest (the certificate puts back what that misses).  Every sample and
+ * vertex of the output still comes
  * from the exact evaluator; a brick holding surface that the surrogate
  * wrongly skipped would be missing from the mesh, which
  * tg_mesher_active_bricks lets a caller audit against a plain export. */

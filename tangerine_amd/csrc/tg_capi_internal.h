@@ -73,6 +73,7 @@ namespace tg {
 // Device vertices of the mesher's last export (3 floats each).
 const float* mesher_vertices(const tg_mesher* m);
 mlp::Net surrogate_net(const tg_surrogate* s);
-// Cull threshold for bricks of half-diagonal r (< 0: not calibrated).
-float surrogate_threshold(const tg_surrogate* s, float r);
+// Cull threshold for bricks of half-diagonal r (< 0: not calibrated); the
+// certified threshold may be lower (skipped bricks are checked exactly).
+float surrogate_threshold(const tg_surrogate* s, float r, bool certified = false);
 }

@@ -423,7 +423,7 @@ int select_tiles(tg_mesher* m, const tg_program* p, const tg_grid* g, const tg_s
 		pa.dx = g->dx; pa.dy = g->dy; pa.dz = g->dz;
 		pa.cbx = ma.cbx; pa.cby = ma.cby; pa.cbz = ma.cbz;
 		pa.radius = radius;
-		pa.threshold = surrogate_threshold(s, radius);
+		pa.threshold = surrogate_threshold(s, radius, certify);
 		pa.flags = m->brick_flags.as<uint8_t>();
 		sel.threshold = pa.threshold;
 		if (!(pa.threshold < pa.net.clamp))

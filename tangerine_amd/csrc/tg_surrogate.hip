@@ -52,6 +52,9 @@ struct tg_surrogate
 	bool calibrated = false;
 	std::vector<float> calib_pred, calib_dist;  // held-out predictions and exact distances
 	float mean_residual = 0.0f;
+	// last threshold per mode (uncertified, certified) and the r it was for;
+	// cleared whenever the calibration changes
+	mutable float cache_r[2] = { -1.0f, -1.0f }, cache_t[2] = { 0.0f, 0.0f };
 	DevBuf params, fourier, adam_m, adam_v, xyz, target, order, partial, loss_partial, counters, loss_log, tmp;
 	tg_surrogate_stats stats{};
 };
@@ -69,15 +72,38 @@ mlp::Net surrogate_net(const tg_surrogate* s)
 	return n;
 }
 
-float surrogate_threshold(const tg_surrogate* s, float r)
+float surrogate_threshold(const tg_surrogate* s, float r, bool certified)
 {
 	if (s->margin >= 0.0f) return r + s->margin;
 	if (!s->calibrated) return -1.0f;
+	const int mode = certified ? 1 : 0;
+	if (s->cache_r[mode] == r) return s->cache_t[mode];
+	// |prediction| at held-out points whose exact |d| <= r: a brick of that
+	// half-diagonal holding surface looks like one of them.  Uncertified, the
+	// largest is the bar; certified, every skipped brick is checked exactly
+	// anyway, so the bar is their 90th percentile (fewer bricks kept for
+	// nothing, the rest put back by the certificate).
+	std::vector<float> near;
 	float t = r;
 	const size_t n = s->calib_pred.size();
 	for (size_t i = 0; i < n; ++i)
-		if (std::fabs(s->calib_dist[i]) <= r) t = std::max(t, std::fabs(s->calib_pred[i]));
-	return t + s->mean_residual;
+		if (std::fabs(s->calib_dist[i]) <= r)
+		{
+			const float a = std::fabs(s->calib_pred[i]);
+			if (certified)
+				near.push_back(a);
+			else
+				t = std::max(t, a);
+		}
+	if (certified && !near.empty())
+	{
+		const size_t k = size_t(0.9 * double(near.size() - 1));
+		std::nth_element(near.begin(), near.begin() + std::ptrdiff_t(k), near.end());
+		t = std::max(t, near[k]);
+	}
+	s->cache_r[mode] = r;
+	s->cache_t[mode] = t + s->mean_residual;
+	return s->cache_t[mode];
 }
 
 }  // namespace tg
@@ -440,6 +466,7 @@ extern "C" int tg_surrogate_fit(tg_surrogate* s, const tg_program* exact, uint64
 	s->mean_residual = float(sum_res / double(calib_samples));
 	s->calibrated = true;
 	s->margin = -1.0f;
+	s->cache_r[0] = s->cache_r[1] = -1.0f;
 	auto t2 = std::chrono::steady_clock::now();
 	tg_surrogate_stats st{};
 	st.sample_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();

@@ -191,7 +191,10 @@ def test_fit_then_culled_export_is_identical(gpu):
     kept = m.brick_flags().astype(bool)
     v1, t1 = m.mesh()
     assert culled["bricks_certified"] >= guess_missed
-    assert culled["bricks_flagged"] == guess["bricks_flagged"] + culled["bricks_certified"]
+    # certified, the surrogate's bar is its 90th percentile, not its largest
+    # near-surface prediction: it proposes no more bricks than uncertified
+    assert culled["cull_threshold"] <= guess["cull_threshold"]
+    assert culled["bricks_flagged"] - culled["bricks_certified"] <= guess["bricks_flagged"]
     assert int(np.count_nonzero(active & ~kept)) == 0
     assert v0.shape == v1.shape and t0.shape == t1.shape
     assert v0.tobytes() == v1.tobytes() and t0.tobytes() == t1.tobytes()
