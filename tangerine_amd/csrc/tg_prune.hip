@@ -128,8 +128,9 @@ int prune_cells(hipStream_t stream, const tg_program* p, const float origin[3], 
 			e = hipMemsetAsync(out.lens.ptr, 0, waves * 64 * 4, stream);
 			if (e != hipSuccess) return hip_fail(e, "length reset");
 			const size_t heights = level_first.size() - 1;
-			// values and decisions upward, then liveness and lengths downward
-			for (size_t step = 0; step < 2 * heights; ++step)
+			// values and decisions upward from height 1 (k_prune_level evaluates
+			// leaf operands in place), then liveness and lengths downward
+			for (size_t step = 1; step < 2 * heights; ++step)
 			{
 				const size_t h = step < heights ? step : 2 * heights - 1 - step;
 				la.liveness = step < heights ? 0 : 1;
