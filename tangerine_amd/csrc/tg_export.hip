@@ -144,8 +144,7 @@ int surface_nets(tg_mesher* m)
 	if (nv)
 	{
 		TG_CHECK_HIP(mesh_list_cubes(a, m->stream), "cube list launch");
-		TG_CHECK_HIP(mesh_vertices(a, nv, m->stream), "vertex launch");
-		TG_CHECK_HIP(mesh_quad_count(a, nv, m->stream), "quad count launch");
+		TG_CHECK_HIP(mesh_vertices(a, nv, m->stream), "vertex launch");  // also each cube's quad count
 		tmp = 0;
 		TG_CHECK_HIP(exclusive_scan_u32(nullptr, nullptr, nv, nullptr, tmp, m->stream), "scan sizing");
 		rc = m->scan_tmp.ensure(tmp);

@@ -39,7 +39,6 @@ hipError_t mesh_classify(const MeshArgs& a, hipStream_t s);
 // quad counts and, after their scan, triangles.
 hipError_t mesh_list_cubes(const MeshArgs& a, hipStream_t s);
 hipError_t mesh_vertices(const MeshArgs& a, uint64_t nverts, hipStream_t s);
-hipError_t mesh_quad_count(const MeshArgs& a, uint64_t nverts, hipStream_t s);
 hipError_t mesh_quad_write(const MeshArgs& a, uint64_t nverts, hipStream_t s);
 hipError_t mesh_pool_to_dense(const MeshArgs& a, float* dense, uint32_t k0, uint32_t nk, hipStream_t s);
 hipError_t mesh_dense_to_pool(const MeshArgs& a, const float* dense, hipStream_t s);

@@ -218,19 +218,11 @@ __device__ __forceinline__ void cube_coords(const MeshArgs& a, uint32_t c, uint3
 	k = r / a.sy;
 }
 
-// Thread per active cube: vertex placement.
-__global__ void __launch_bounds__(256) k_vertices(MeshArgs a, uint64_t nverts)
-{
-	const uint64_t id = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-	if (id >= nverts) return;
-	uint32_t i, j, k;
-	cube_coords(a, a.cubes[id], i, j, k);
-	place_vertex(a, i, j, k, a.vertices + size_t(id) * 3);
-}
-
-// Quads of one active cube: writes up to 6 triangles when `out` is set and
-// returns the quad count.
-__device__ __forceinline__ uint32_t cube_quads(const MeshArgs& a, uint32_t i, uint32_t j, uint32_t k, uint32_t* out)
+// Quads of one active cube (vertex `self`: active cubes are listed in vertex
+// order): writes up to 6 triangles when `out` is set and returns the quad
+// count.
+__device__ __forceinline__ uint32_t cube_quads(const MeshArgs& a, uint32_t i, uint32_t j, uint32_t k, uint32_t* out,
+	uint32_t self)
 {
 	if (i == 0 || j == 0 || k == 0) return 0;
 	// neighbour cubes 0..5 (surface_nets.cpp:415-423)
@@ -255,7 +247,7 @@ __device__ __forceinline__ uint32_t cube_quads(const MeshArgs& a, uint32_t i, ui
 #pragma unroll
 			for (int t = 0; t < 3; ++t) n[t] = vertex_id(a, nx[quad[q][t]], ny[quad[q][t]], nz[quad[q][t]]);
 			const bool fwd = second > first;
-			const uint32_t v0 = vertex_id(a, i, j, k);
+			const uint32_t v0 = self;
 			const uint32_t v1 = fwd ? n[0] : n[2], v2 = n[1], v3 = fwd ? n[2] : n[0];
 			uint32_t* tri = out + size_t(count) * 6;
 			tri[0] = v0; tri[1] = v1; tri[2] = v2;
@@ -266,23 +258,26 @@ __device__ __forceinline__ uint32_t cube_quads(const MeshArgs& a, uint32_t i, ui
 	return count;
 }
 
-// Thread per active cube: quads it emits, then (after their scan) the quads.
-__global__ void __launch_bounds__(256) k_quad_count(MeshArgs a, uint64_t nverts)
+// Thread per active cube: vertex placement and the number of quads the cube
+// emits (scanned before k_quad_write).
+__global__ void __launch_bounds__(256) k_vertices(MeshArgs a, uint64_t nverts)
 {
 	const uint64_t id = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (id >= nverts) return;
 	uint32_t i, j, k;
 	cube_coords(a, a.cubes[id], i, j, k);
-	a.cube_quads[id] = cube_quads(a, i, j, k, nullptr);
+	place_vertex(a, i, j, k, a.vertices + size_t(id) * 3);
+	a.cube_quads[id] = cube_quads(a, i, j, k, nullptr, uint32_t(id));
 }
 
+// Thread per active cube: its quads at the scanned base.
 __global__ void __launch_bounds__(256) k_quad_write(MeshArgs a, uint64_t nverts)
 {
 	const uint64_t id = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (id >= nverts) return;
 	uint32_t i, j, k;
 	cube_coords(a, a.cubes[id], i, j, k);
-	cube_quads(a, i, j, k, a.triangles + size_t(a.cube_qbase[id]) * 6);
+	cube_quads(a, i, j, k, a.triangles + size_t(a.cube_qbase[id]) * 6, uint32_t(id));
 }
 
 // Pool tiles -> dense x-fastest grid (for copying samples back to the host).
@@ -355,13 +350,6 @@ hipError_t mesh_vertices(const MeshArgs& a, uint64_t nverts, hipStream_t s)
 {
 	if (nverts == 0) return hipSuccess;
 	hipLaunchKernelGGL(k_vertices, dim3(blocks_for(nverts, 256)), dim3(256), 0, s, a, nverts);
-	return hipGetLastError();
-}
-
-hipError_t mesh_quad_count(const MeshArgs& a, uint64_t nverts, hipStream_t s)
-{
-	if (nverts == 0) return hipSuccess;
-	hipLaunchKernelGGL(k_quad_count, dim3(blocks_for(nverts, 256)), dim3(256), 0, s, a, nverts);
 	return hipGetLastError();
 }
 
