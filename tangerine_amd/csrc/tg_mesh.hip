@@ -249,9 +249,12 @@ __device__ __forceinline__ uint32_t cube_quads(const MeshArgs& a, uint32_t i, ui
 			const bool fwd = second > first;
 			const uint32_t v0 = self;
 			const uint32_t v1 = fwd ? n[0] : n[2], v2 = n[1], v3 = fwd ? n[2] : n[0];
-			uint32_t* tri = out + size_t(count) * 6;
-			tri[0] = v0; tri[1] = v1; tri[2] = v2;
-			tri[3] = v0; tri[4] = v2; tri[5] = v3;
+			// two triangles (v0, v1, v2), (v0, v2, v3): 24 bytes at an 8-byte
+			// aligned place (quad bases are whole quads), as three 8-byte stores
+			uint2* tri = reinterpret_cast<uint2*>(out + size_t(count) * 6);
+			tri[0] = make_uint2(v0, v1);
+			tri[1] = make_uint2(v2, v0);
+			tri[2] = make_uint2(v2, v3);
 		}
 		++count;
 	}
