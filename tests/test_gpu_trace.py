@@ -104,6 +104,30 @@ def test_render_bit_identical_to_oracle(gpu, oracle, prune):
     assert st["hits"] == hits.sum() and st["rays"] == 160 * 90
 
 
+def test_render_full_hd_sampled(gpu, oracle):
+    """The full 1920x1080 render of the same tree (2,073,600 rays, at most
+    256 steps, eps 1e-4, far 40): 5,000 seeded pixels' t, step count and
+    normal bit-identical to the oracle's march from the same eye."""
+    from tangerine_amd import scenes, trace
+    spec = scenes.random_csg_spec(1, 256)
+    eye = trace.seeded_eye(3)
+    node, prog, cam, st, t, n, steps = _render_case(oracle, spec, eye, 1920, 1080, True)
+    assert st["rays"] == 1920 * 1080 and t.size == 1920 * 1080
+    words = node.compile()
+    dirs = trace.camera_directions(cam)
+    pick = np.random.default_rng(11).choice(dirs.shape[0], 5000, replace=False)
+    d = dirs[pick]
+    starts = np.repeat(np.array([cam.eye[:]], dtype=np.float32), d.shape[0], axis=0)
+    oh, ot, osteps = oracle.raymarch_far(words, starts, d, 256, 1e-4, 40.0)
+    tt, nn, ss = t.reshape(-1)[pick], n.reshape(-1, 3)[pick], steps.reshape(-1)[pick]
+    hits = np.isfinite(tt)
+    assert np.array_equal(hits, oh) and 0.05 < hits.mean() < 0.95
+    assert np.array_equal(bits(tt), bits(ot))
+    assert np.array_equal(ss.astype(np.int32), osteps)
+    p = (d[hits] * tt[hits][:, None] + starts[hits]).astype(np.float32)
+    assert np.array_equal(bits(nn[hits]), bits(oracle.gradient_central(words, p, 1e-3)))
+
+
 def test_render_fast_close_to_exact(gpu, oracle):
     from tangerine_amd import scenes, trace
     spec = scenes.random_csg_spec(1, 256)
