@@ -24,8 +24,6 @@ struct MeshArgs
 	uint64_t* active;
 	uint32_t* word_count;
 	uint32_t* word_base;
-	uint32_t* quad_count;
-	uint32_t* quad_base;
 	uint32_t* cubes;              // active cubes in vertex-id order: i + sx (j + sy k)
 	uint32_t* cube_quads;         // quads per active cube, then their exclusive scan
 	uint32_t* cube_qbase;
