@@ -111,7 +111,7 @@ def cpu_baseline(spec, lo, hi, cells, budget_s: float, threads: int) -> dict:
 
 def surrogate_leg(mesher, prog, wl, origin, step, cells, args) -> dict:
     """Fit the brick-culling surrogate, time culled exports of the same grid,
-    and judge them against the plain export that ran last."""
+    and check them against the plain export that ran last."""
     import hashlib
     from tangerine_amd.surrogate import Surrogate
 
@@ -259,8 +259,8 @@ def main(argv=None) -> int:
         return 0
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; run with --gpus 1 (multi-GPU sharding is "
-                                   "described in DESIGN.md but not built)"}))
+        print(json.dumps({"error": "this benchmark measures one GPU; run with --gpus 1 (sharding across "
+                                   "GPUs is not built)"}))
         return 2
     if int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "single-GPU benchmark; do not launch under torch.distributed"}))

@@ -1,5 +1,5 @@
 """Learned brick culling: the surrogate's kernels against the CPU oracle, and
-culled export judged by the plain export.
+culled export checked against the plain export.
 
 Bars:
   * forward: device predictions against the oracle's BF16-rounding variant,
@@ -11,7 +11,7 @@ Bars:
     certificate alone): vertices and triangles byte-identical to the plain
     exact export of the same grid, no brick holding an active cube left out,
     and fewer tiles evaluated than the plain export; the surrogate alone
-    (uncertified) is audited against the same judge.
+    (uncertified) is audited against the same plain export.
 """
 
 import numpy as np
