@@ -22,7 +22,7 @@
 //   RayMarch                 tangerine/sdf_evaluator.cpp:336-353
 //   serial surface nets      third_party/naive-surface-nets/src/surface_nets.cpp:30-580
 //
-// Functions this build adds (no reference counterpart) and how they are pinned:
+// Functions with no reference counterpart and how they are pinned:
 //   tgo_gradient_central     axis-aligned central differences; pinned by closed-form
 //                            normals of spheres/boxes in tests/test_oracle.py.
 //   tgo_raymarch_far         RayMarch with a far cutoff; pinned by ray-sphere closed form.
