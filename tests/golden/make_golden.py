@@ -2,7 +2,8 @@
 sources (oracle/_ref/libtgref.so, built by `make -C oracle` from
 /root/reference).  Run in the build container only:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py                   # every fixture
+    python tests/golden/make_golden.py --only-big-tree   # points_rand2048_s1.npz only
 
 Fixtures are data only: inputs (specs, points, grids) and the reference's
 outputs for them.  Every array is saved with numpy (no pickles).
@@ -57,10 +58,45 @@ def spec_array(spec) -> np.ndarray:
     return np.array(json.dumps(spec))
 
 
+def points_fixture(ref, name, spec, ext, rng) -> dict:
+    """Point queries of one tree: tree walk (SDFNode::Eval) and interpreter
+    values at uniform points, gradients, and ray marches."""
+    h = ref.build(spec)
+    words = ref.compile(h)
+    pts = rng.uniform(-ext, ext, size=(4096, 3)).astype(np.float32)
+    tree_d = ref.eval(h, pts)
+    interp_d = ref.interp_eval(h, pts)
+    gpts = pts[:512]
+    grad = ref.gradient(h, gpts)
+    starts = rng.uniform(-2 * ext, 2 * ext, size=(512, 3)).astype(np.float32)
+    dirs = rng.normal(size=(512, 3)).astype(np.float32)
+    hit, travel, pos = ref.raymarch(h, starts, dirs, 100, 1e-3)
+    np.savez_compressed(
+        OUT / f"points_{name}.npz", spec=spec_array(spec), words=words, points=pts, tree_eval=tree_d,
+        interp_eval=interp_d, bounds=ref.bounds(h), leaf_count=np.int64(ref.fn("leaf_count")(h)),
+        stack_size=np.int64(ref.fn("stack_size")(h)), grad_points=gpts, gradient=grad,
+        ray_start=starts, ray_dir=dirs, ray_hit=hit, ray_travel=travel, ray_pos=pos)
+    out = {"words": int(words.size), "stack": int(ref.fn("stack_size")(h))}
+    ref.free(h)
+    return out
+
+
+def big_tree_fixture(ref, summary) -> None:
+    """The 2048-primitive tree of the 1024^3 export (seed 1) over [-8,8]^3:
+    tree-walk values at uniform points and at points near its surface (ray
+    hits), with its own generator so the other fixtures stay as they are."""
+    summary["rand2048_s1"] = points_fixture(ref, "rand2048_s1", scenes.random_csg_spec(1, 2048), 8.0,
+                                            np.random.default_rng(20261203))
+
+
 def main() -> int:
     ref = Reference()
+    summary = json.loads((OUT / "summary.json").read_text()) if (OUT / "summary.json").exists() else {}
+    if "--only-big-tree" in sys.argv:
+        big_tree_fixture(ref, summary)
+        (OUT / "summary.json").write_text(json.dumps(summary, indent=1) + "\n")
+        return 0
     rng = np.random.default_rng(20240601)
-    summary = {}
 
     # ---- point queries for three trees --------------------------------------
     trees = {
@@ -69,23 +105,7 @@ def main() -> int:
         "rand256_s1": (scenes.random_csg_spec(1, 256), 6.0),
     }
     for name, (spec, ext) in trees.items():
-        h = ref.build(spec)
-        words = ref.compile(h)
-        pts = rng.uniform(-ext, ext, size=(4096, 3)).astype(np.float32)
-        tree_d = ref.eval(h, pts)
-        interp_d = ref.interp_eval(h, pts)
-        gpts = pts[:512]
-        grad = ref.gradient(h, gpts)
-        starts = rng.uniform(-2 * ext, 2 * ext, size=(512, 3)).astype(np.float32)
-        dirs = rng.normal(size=(512, 3)).astype(np.float32)
-        hit, travel, pos = ref.raymarch(h, starts, dirs, 100, 1e-3)
-        np.savez_compressed(
-            OUT / f"points_{name}.npz", spec=spec_array(spec), words=words, points=pts, tree_eval=tree_d,
-            interp_eval=interp_d, bounds=ref.bounds(h), leaf_count=np.int64(ref.fn("leaf_count")(h)),
-            stack_size=np.int64(ref.fn("stack_size")(h)), grad_points=gpts, gradient=grad,
-            ray_start=starts, ray_dir=dirs, ray_hit=hit, ray_travel=travel, ray_pos=pos)
-        summary[name] = {"words": int(words.size), "stack": int(ref.fn("stack_size")(h))}
-        ref.free(h)
+        summary[name] = points_fixture(ref, name, spec, ext, rng)
 
     # ---- Clip outputs (program words of the clipped tree) --------------------
     h = ref.build(scenes.config1_spec())
@@ -172,6 +192,7 @@ def main() -> int:
         np.savez_compressed(OUT / "vox_write_multi.npz", size=size, cells=cells, color_index=np.int64(200), vox_bytes=data)
         summary["vox_write_multi"] = {"cells": int(cells.size), "bytes": int(data.size)}
 
+    big_tree_fixture(ref, summary)
     (OUT / "summary.json").write_text(json.dumps(summary, indent=1) + "\n")
     print(json.dumps(summary, indent=1))
     return 0

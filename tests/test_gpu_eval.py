@@ -19,7 +19,8 @@ from conftest import canonical_faces, golden_spec, grid_points, load_golden, sha
 
 pytestmark = pytest.mark.gpu
 
-TREES = [("points_config1.npz", 5.0), ("points_rand16_s5.npz", 12.0), ("points_rand256_s1.npz", 12.0)]
+TREES = [("points_config1.npz", 5.0), ("points_rand16_s5.npz", 12.0), ("points_rand256_s1.npz", 12.0),
+         ("points_rand2048_s1.npz", 16.0)]
 
 
 def bits(a):
@@ -122,7 +123,8 @@ def test_sqrt_special_arguments_bit_identical(gpu, oracle):
         assert np.array_equal(d.view(np.uint32), ref.view(np.uint32))
 
 
-@pytest.mark.parametrize("name", ["points_config1.npz", "points_rand16_s5.npz", "points_rand256_s1.npz"])
+@pytest.mark.parametrize("name", ["points_config1.npz", "points_rand16_s5.npz", "points_rand256_s1.npz",
+                                  "points_rand2048_s1.npz"])
 def test_non_finite_points_match_oracle(gpu, oracle, name):
     """Infinite and NaN coordinates (where rays that march off to infinity
     end up): the device gives the oracle's value wherever that is a number and

@@ -32,7 +32,7 @@ def unit(d):
 
 
 @pytest.mark.parametrize("name,extent", [("points_config1.npz", 5.0), ("points_rand16_s5.npz", 12.0),
-                                         ("points_rand256_s1.npz", 12.0)])
+                                         ("points_rand256_s1.npz", 12.0), ("points_rand2048_s1.npz", 16.0)])
 def test_raymarch_matches_oracle_and_reference(gpu, oracle, name, extent):
     from tangerine_amd.sdf import Program
     from tangerine_amd.trace import raymarch
@@ -52,7 +52,8 @@ def test_raymarch_matches_oracle_and_reference(gpu, oracle, name, extent):
     assert np.all(np.abs(pos[both] - g["ray_pos"][both]) <= 1e-5 * extent * 100)
 
 
-@pytest.mark.parametrize("name", ["points_config1.npz", "points_rand16_s5.npz", "points_rand256_s1.npz"])
+@pytest.mark.parametrize("name", ["points_config1.npz", "points_rand16_s5.npz", "points_rand256_s1.npz",
+                                  "points_rand2048_s1.npz"])
 def test_gradients(gpu, oracle, name):
     from tangerine_amd.sdf import Program
     from tangerine_amd.trace import gradient

@@ -32,7 +32,8 @@ def test_library_exports_every_declared_symbol():
     assert set(names) <= set(_lib.SIGNATURES)
 
 
-@pytest.mark.parametrize("name", ["points_config1.npz", "points_rand16_s5.npz", "points_rand256_s1.npz"])
+@pytest.mark.parametrize("name", ["points_config1.npz", "points_rand16_s5.npz", "points_rand256_s1.npz",
+                                  "points_rand2048_s1.npz"])
 def test_flattener_words_match_reference(name):
     from tangerine_amd import scenes
     g = load_golden(name)

@@ -9,7 +9,7 @@ import pytest
 
 from conftest import canonical_faces, golden_spec, grid_points, load_golden, sha
 
-TREES = ["points_config1.npz", "points_rand16_s5.npz", "points_rand256_s1.npz"]
+TREES = ["points_config1.npz", "points_rand16_s5.npz", "points_rand256_s1.npz", "points_rand2048_s1.npz"]
 
 
 def bits(a):
