@@ -218,6 +218,19 @@ int tg_gradient_points(const tg_program* program, const float* xyz, uint64_t cou
 int tg_raymarch(const tg_program* program, const float* starts, const float* dirs, uint64_t count,
                 int max_steps, float epsilon, float far_distance, uint32_t flags,
                 uint8_t* hit, float* travel, float* position);
+/* tg_raymarch without any device allocation (no reference counterpart; for
+ * callers that cannot allocate: tight per-frame query loops such as the Lua
+ * ray_cast / magnet bindings, serving loops).  Same arguments and results as
+ * tg_raymarch; rays are staged through a fixed device array of the library
+ * (TG_RAYMARCH_STATIC_MAX rays, 41 bytes each, held by every device the
+ * library runs on until the process ends).  count > TG_RAYMARCH_STATIC_MAX
+ * returns TG_ERR_LIMIT.  Every call on one device shares that array: calls
+ * that overlap on one device (from several threads) race and must be
+ * serialised by the caller. */
+#define TG_RAYMARCH_STATIC_MAX 65536u
+int tg_raymarch_static(const tg_program* program, const float* starts, const float* dirs, uint64_t count,
+                       int max_steps, float epsilon, float far_distance, uint32_t flags,
+                       uint8_t* hit, float* travel, float* position);
 /* ---- Lua queries (tangerine/lua_sdf.cpp:386-559, sdf_model.cpp:150-264) --- */
 /* pivot_towards (LuaPivotTowards, lua_sdf.cpp:452-559) for `count` queries:
  * pivots, headings, downs: 3 floats each; the scalar arguments as in Lua

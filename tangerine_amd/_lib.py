@@ -124,6 +124,7 @@ SIGNATURES = {
     "tg_eval_points_device": (C.c_int, [_p, _p, _u64, _p, _u32, _p]),
     "tg_gradient_points": (C.c_int, [_p, _fp, _u64, _f, C.c_int, _fp, _u32]),
     "tg_raymarch": (C.c_int, [_p, _fp, _fp, _u64, C.c_int, _f, _f, _u32, C.POINTER(C.c_uint8), _fp, _fp]),
+    "tg_raymarch_static": (C.c_int, [_p, _fp, _fp, _u64, C.c_int, _f, _f, _u32, C.POINTER(C.c_uint8), _fp, _fp]),
     "tg_pivot_towards": (C.c_int, [_p, _u64, _fp, _fp, _fp, _f, _f, _f, C.c_int, _fp, _fp, C.POINTER(C.c_int32), _u32]),
     "tg_pick": (C.c_int, [C.POINTER(_p), _fp, _u32, _fp, _fp, _u64, _u32, C.POINTER(C.c_int32), _fp, _fp]),
     "tg_camera_look_at": (C.c_int, [C.POINTER(TgCamera), _fp, _fp, _fp, _f, _u32, _u32]),

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "../../include/tangerine_gpu.h"
@@ -255,6 +256,73 @@ extern "C" int tg_tracer_stats(const tg_tracer* tr, tg_trace_stats* out)
 	return TG_OK;
 }
 
+namespace {
+
+// One batched march of `count` rays whose inputs and outputs are device
+// arrays already holding / sized for them; results copied to the host arrays.
+int raymarch_staged(const tg_program* p, const float* starts, const float* dirs, uint64_t count, int max_steps,
+	float eps, float far_distance, uint32_t flags, uint8_t* hit, float* travel, float* position, float* ds, float* dd,
+	float* dt, float* dp, uint8_t* dh)
+{
+	TG_CHECK_HIP(hipMemcpy(ds, starts, count * 12, hipMemcpyHostToDevice), "ray upload");
+	TG_CHECK_HIP(hipMemcpy(dd, dirs, count * 12, hipMemcpyHostToDevice), "ray upload");
+	TraceArgs ta;
+	base_args(ta, p);
+	ta.camera = 0;
+	ta.starts = ds;
+	ta.dirs = dd;
+	ta.nrays = uint32_t(count);
+	ta.max_steps = max_steps;
+	ta.eps = eps;
+	ta.far_ = far_distance;
+	ta.h = 0.0f;
+	ta.t_out = dt;
+	ta.p_out = dp;
+	ta.hit_out = dh;
+	hipError_t e = (flags & TG_FLAG_FAST) ? fast::launch_trace(ta, nullptr) : exact::launch_trace(ta, nullptr);
+	if (e != hipSuccess) return hip_fail(e, "raymarch launch");
+	TG_CHECK_HIP(hipMemcpy(travel, dt, count * 4, hipMemcpyDeviceToHost), "travel download");
+	if (position) TG_CHECK_HIP(hipMemcpy(position, dp, count * 12, hipMemcpyDeviceToHost), "position download");
+	if (hit) TG_CHECK_HIP(hipMemcpy(hit, dh, count, hipMemcpyDeviceToHost), "hit download");
+	return TG_OK;
+}
+
+constexpr uint64_t kStaticRays = TG_RAYMARCH_STATIC_MAX;
+constexpr size_t kStaticBytesPerRay = 41;  // start 12, direction 12, travel 4, position 12, hit 1
+
+}  // namespace
+
+// Staging of tg_raymarch_static: one copy per device, for the whole process.
+// In host code this name is only a handle for hipGetSymbolAddress; kernels
+// and copies take the device address that call returns.
+static __device__ __align__(256) unsigned char scratch[kStaticBytesPerRay * kStaticRays];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per device.
+int static_scratch(void** out)
+{
+	int dev = 0;
+	TG_CHECK_HIP(hipGetDevice(&dev), "hipGetDevice");
+	if (dev < 0 || dev >= kMaxDevices) return fail(TG_ERR_LIMIT, "tg_raymarch_static: device index beyond 63");
+	std::lock_guard<std::mutex> lock(g_scratch_mutex);
+	if (!g_scratch_ptr[dev])
+	{
+		void* ptr = nullptr;
+		TG_CHECK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+		if (!ptr) return fail(TG_ERR_DEVICE, "tg_raymarch_static: no device address for the staging array");
+		g_scratch_ptr[dev] = ptr;
+	}
+	*out = g_scratch_ptr[dev];
+	return TG_OK;
+}
+
+}  // namespace
+
 extern "C" int tg_raymarch(const tg_program* p, const float* starts, const float* dirs, uint64_t count, int max_steps,
 	float eps, float far_distance, uint32_t flags, uint8_t* hit, float* travel, float* position)
 {
@@ -271,27 +339,29 @@ extern "C" int tg_raymarch(const tg_program* p, const float* starts, const float
 	if (!rc) rc = dp.ensure(count * 12);
 	if (!rc) rc = dh.ensure(count);
 	if (rc) return rc;
-	TG_CHECK_HIP(hipMemcpy(ds.ptr, starts, count * 12, hipMemcpyHostToDevice), "ray upload");
-	TG_CHECK_HIP(hipMemcpy(dd.ptr, dirs, count * 12, hipMemcpyHostToDevice), "ray upload");
-	TraceArgs ta;
-	base_args(ta, p);
-	ta.camera = 0;
-	ta.starts = ds.as<const float>();
-	ta.dirs = dd.as<const float>();
-	ta.nrays = uint32_t(count);
-	ta.max_steps = max_steps;
-	ta.eps = eps;
-	ta.far_ = far_distance;
-	ta.h = 0.0f;
-	ta.t_out = dt.as<float>();
-	ta.p_out = dp.as<float>();
-	ta.hit_out = dh.as<uint8_t>();
-	hipError_t e = (flags & TG_FLAG_FAST) ? fast::launch_trace(ta, nullptr) : exact::launch_trace(ta, nullptr);
-	if (e != hipSuccess) return hip_fail(e, "raymarch launch");
-	TG_CHECK_HIP(hipMemcpy(travel, dt.ptr, count * 4, hipMemcpyDeviceToHost), "travel download");
-	if (position) TG_CHECK_HIP(hipMemcpy(position, dp.ptr, count * 12, hipMemcpyDeviceToHost), "position download");
-	if (hit) TG_CHECK_HIP(hipMemcpy(hit, dh.ptr, count, hipMemcpyDeviceToHost), "hit download");
-	return TG_OK;
+	return raymarch_staged(p, starts, dirs, count, max_steps, eps, far_distance, flags, hit, travel, position,
+		ds.as<float>(), dd.as<float>(), dt.as<float>(), dp.as<float>(), dh.as<uint8_t>());
+}
+
+extern "C" int tg_raymarch_static(const tg_program* p, const float* starts, const float* dirs, uint64_t count,
+	int max_steps, float eps, float far_distance, uint32_t flags, uint8_t* hit, float* travel, float* position)
+{
+	if (!p || (count && (!starts || !dirs || !travel))) return fail(TG_ERR_INVALID, "tg_raymarch_static: null argument");
+	if (count == 0) return TG_OK;
+	if (count > kStaticRays) return fail(TG_ERR_LIMIT, "tg_raymarch_static: more than TG_RAYMARCH_STATIC_MAX rays");
+	int rc = ensure_device();
+	if (rc) return rc;
+	void* base = nullptr;
+	rc = static_scratch(&base);
+	if (rc) return rc;
+	unsigned char* b = static_cast<unsigned char*>(base);
+	float* ds = reinterpret_cast<float*>(b);
+	float* dd = reinterpret_cast<float*>(b + 12 * kStaticRays);
+	float* dt = reinterpret_cast<float*>(b + 24 * kStaticRays);
+	float* dp = reinterpret_cast<float*>(b + 28 * kStaticRays);
+	uint8_t* dh = b + 40 * kStaticRays;
+	return raymarch_staged(p, starts, dirs, count, max_steps, eps, far_distance, flags, hit, travel, position, ds, dd,
+		dt, dp, dh);
 }
 
 extern "C" int tg_gradient_points(const tg_program* p, const float* xyz, uint64_t count, float h, int mode, float* out,
