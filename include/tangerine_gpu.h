@@ -103,6 +103,9 @@ typedef struct tg_program_info
 	uint32_t set_ops;
 	uint32_t lipschitz;        /* 1 if every primitive is a distance bound (no ellipsoid) */
 	double flops_per_eval;     /* algorithmic FP32 work of one full-program evaluation */
+	float leaf_extent;         /* largest world-space brush size (field scale x (|frame translation| + sum
+	                            * |parameters|)); sets the rounding slack of pruning and certification */
+	float reserved;
 } tg_program_info;
 
 typedef struct tg_export_stats

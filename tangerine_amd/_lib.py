@@ -34,7 +34,8 @@ class TgGrid(C.Structure):
 
 class TgProgramInfo(C.Structure):
     _fields_ = [("instructions", _u64), ("max_stack", _u32), ("brushes", _u32), ("set_ops", _u32),
-                ("lipschitz", _u32), ("flops_per_eval", C.c_double)]
+                ("lipschitz", _u32), ("flops_per_eval", C.c_double), ("leaf_extent", C.c_float),
+                ("reserved", C.c_float)]
 
 
 class TgExportStats(C.Structure):

@@ -345,6 +345,8 @@ static void fill_info(const DecodedProgram& d, tg_program_info* out)
 	out->set_ops = d.set_ops;
 	out->lipschitz = d.lipschitz ? 1u : 0u;
 	out->flops_per_eval = d.flops;
+	out->leaf_extent = d.leaf_extent;
+	out->reserved = 0.0f;
 }
 
 extern "C" int tg_program_info_get(const tg_program* p, tg_program_info* out)

@@ -258,6 +258,20 @@ int prune_blocks(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t f
 	return prune_cells(m->stream, p, origin, cell, radius, counts, (flags & TG_FLAG_FAST) != 0, m->blocks);
 }
 
+// Largest |p| over the grid's samples (the corners of its box).
+float grid_reach(const tg_grid* g)
+{
+	const float o[3] = { g->x, g->y, g->z }, d[3] = { g->dx, g->dy, g->dz };
+	const uint64_t n[3] = { g->sx, g->sy, g->sz };
+	double s = 0.0;
+	for (int i = 0; i < 3; ++i)
+	{
+		const double a = std::fabs(double(o[i])), b = std::fabs(double(o[i]) + double(n[i]) * double(d[i]));
+		s += std::max(a, b) * std::max(a, b);
+	}
+	return float(std::sqrt(s));
+}
+
 // Block-kernel arguments of one grid (after prune_blocks).
 BlockEvalArgs block_args(tg_mesher* m, const tg_program* p, const tg_grid* g, const uint64_t* tile_mask,
 	const uint32_t* block_list, uint32_t n_listed, unsigned long long* work)
@@ -291,6 +305,7 @@ BlockEvalArgs block_args(tg_mesher* m, const tg_program* p, const tg_grid* g, co
 	ba.max_len = m->blocks.max_len;
 	ba.waves = m->waves;
 	ba.min_prune_len = m->min_prune_len;
+	ba.slack = rounding_slack(p->decoded, grid_reach(g));
 	return ba;
 }
 
@@ -447,7 +462,7 @@ int select_tiles(tg_mesher* m, const tg_program* p, const tg_grid* g, const tg_s
 		ca.dx = g->dx; ca.dy = g->dy; ca.dz = g->dz;
 		ca.bx = bx; ca.by = by; ca.bz = bz;
 		ca.cbx = ma.cbx; ca.cby = ma.cby; ca.cbz = ma.cbz;
-		ca.radius = radius;
+		ca.radius = radius * (1.0f + 1.0e-5f) + rounding_slack(p->decoded, grid_reach(g));
 		ca.flags = m->brick_flags.as<uint8_t>();
 		ca.counters = ctr + 2;
 		ca.slots = std::max<uint32_t>(1, std::max(blocks ? m->blocks.max_stack : 0u, p->decoded.max_stack) - 1);

@@ -46,13 +46,20 @@ struct GridArgs
 // `rec`" (or negate the top of stack), and where its operands' roots are.
 struct Entry
 {
-	uint32_t rec;   // record index (bits 0..21) | transform kind << 22 | opcode << 24 (OP_NEGATE: no record)
+	uint32_t rec;   // record index (bits 0..21) | transform kind << 22 | opcode << 24 (6 bits; OP_NEGATE: no
+	                // record) | kEntryPlusZero
 	uint32_t l;     // binary: position of the left operand's root; unary: operand root
 	uint32_t r;     // binary: position of the right operand's root
 	uint32_t ord;   // binary: ordinal among the program's binary entries
 };
 
 constexpr uint32_t kRecMask = 0x003fffffu;
+// Entry flag: after this entry, top += +0.0.  Pruning sets it on the kept
+// operand's root when a BlendInter or BlendDiff drops one operand: the
+// reference adds the blend's zero term in double (+0.0), which turns a kept
+// -0.0 into +0.0 (BlendUnion subtracts it, which keeps -0.0).
+constexpr uint32_t kEntryPlusZero = 1u << 31;
+__host__ __device__ inline uint32_t entry_op(uint32_t rec) { return (rec >> 24) & 0x3fu; }
 constexpr int kBlockTiles = 4;                 // tiles per block edge
 constexpr int kTilesPerBlock = 64;             // 4^3 tiles = one lane each
 constexpr int kBlockSamples = kTile * kBlockTiles;
@@ -146,6 +153,7 @@ struct BlockEvalArgs
 	uint32_t max_binary;         // LDS tables: binary entries / entries of any block program
 	uint32_t max_len;
 	int waves;                   // waves per workgroup
+	float slack;                 // rounding slack of the per-tile pruning decisions (rounding_slack)
 };
 
 // Persistent batched evaluation of many trees' blocks (one queue item = one
@@ -177,7 +185,7 @@ struct CertifyArgs
 	float gx, gy, gz, dx, dy, dz;
 	uint32_t bx, by, bz;         // blocks per axis
 	uint32_t cbx, cby, cbz;      // bricks per axis
-	float radius;                // brick half-diagonal
+	float radius;                // brick half-diagonal (1 + 1e-5) + rounding slack (rounding_slack)
 	uint8_t* flags;
 	unsigned long long* counters;  // [0] bricks newly flagged, [1] cost units executed
 	uint32_t slots;

@@ -87,6 +87,31 @@ double instr_flops(const Instr& in)
 	}
 }
 
+// Pruning drops an operand of a set operator when, at a cell centre c, the
+// computed operand values a, b satisfy (for union) b - a >= 2 (r' + 1e-5
+// (|a| + |b|)), r' = r (1 + 1e-5) + slack, r the cell radius.  Both fields
+// are 1-Lipschitz (the decoder checks the transform norm to 1e-5), so their
+// exact values move by at most r (1 + 1e-5) inside the cell; the computed
+// values differ from the exact ones by the rounding error E(p) of the
+// interpreter, which holds at c and at every sample p.  Dropping is exact when
+// 2 slack >= E_a(c) + E_a(p) + E_b(c) + E_b(p).  Per brush, the float
+// arithmetic (a 3x4 transform in four roundings per component, then at most
+// a few dozen operations of the primitive, the divides and square roots
+// included) errs by at most K u (|p| + leaf extent) in world units, u =
+// 2^-24, with K = 64 taken generously; set operators propagate the larger
+// operand error (each is 1-Lipschitz in either operand) plus u |value|, and
+// flate / field scale add u |value|, which the value-proportional 1e-5 term
+// covers for any tree depth below 160.  Hence slack = 2 K u (pmax + extent),
+// with pmax bounding |p| over the domain; it never goes below 2e-4.  The
+// oracle's float64 interpreter measures the per-brush constant on random and
+// adversarial trees (tests/test_oracle.py).
+float rounding_slack(const DecodedProgram& d, float pmax)
+{
+	const double u = std::ldexp(1.0, -24), K = 64.0;
+	const double s = 2.0 * K * u * (double(std::fabs(pmax)) + double(d.leaf_extent));
+	return float(std::max(s, 2.0e-4));
+}
+
 bool decode_words(const uint32_t* w, size_t n, DecodedProgram& out, std::string& err)
 {
 	out = DecodedProgram();
@@ -96,6 +121,7 @@ bool decode_words(const uint32_t* w, size_t n, DecodedProgram& out, std::string&
 	float pending[12] = { 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0 };
 	std::vector<uint32_t> starts;  // subtree start record per stack entry
 	double last_brush_norm = 1.0;  // transform norm of the brush just emitted
+	double last_brush_extent = 0.0;  // its extent in its own frame (scaled by a following ScaleField)
 	size_t pc = 0;
 	bool stopped = false;
 	while (pc < n)
@@ -151,6 +177,14 @@ bool decode_words(const uint32_t* w, size_t n, DecodedProgram& out, std::string&
 			}
 			// checked at the next record, which may be the ScaleField undoing a scale
 			last_brush_norm = (pending_kind == XF_MATRIX) ? spectral_norm3(pending) : 1.0;
+			{
+				double mag = std::sqrt(double(pending[3]) * pending[3] + double(pending[7]) * pending[7] +
+				                       double(pending[11]) * pending[11]);
+				for (int i = 0; i < np; ++i) mag += std::fabs(double(as_float(w[pc - np + i])));
+				if (op == OP_CONE) mag += std::fabs(double(as_float(w[pc - np])) * as_float(w[pc - np + 1]));
+				last_brush_extent = mag;
+				out.leaf_extent = std::max(out.leaf_extent, float(mag));
+			}
 			out.max_stack = std::max<uint32_t>(out.max_stack, uint32_t(starts.size()));
 			pending_kind = XF_NONE;
 			const float ident[12] = { 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0 };
@@ -213,6 +247,8 @@ bool decode_words(const uint32_t* w, size_t n, DecodedProgram& out, std::string&
 				// field scale s after a transform of norm n: Lipschitz n * |s|
 				const double ls = last_brush_norm * std::fabs(double(as_float(u.w[1])));
 				if (ls > 1.0 + 1e-5) out.lipschitz = false;
+				out.leaf_extent = std::max(out.leaf_extent,
+				                           float(last_brush_extent * std::fabs(double(as_float(u.w[1])))));
 				last_brush_norm = 1.0;
 			}
 			out.instrs.push_back(u);

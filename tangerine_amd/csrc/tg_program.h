@@ -25,12 +25,20 @@ struct DecodedProgram
 	uint32_t set_ops = 0;
 	bool lipschitz = true;       // false if any ellipsoid (not a distance bound)
 	double flops = 0.0;          // algorithmic FP32 work per evaluation (cost table)
+	// Largest world-space size of a brush: its field scale times (|its frame
+	// translation| + the sum of its parameter magnitudes); bounds the operand
+	// magnitudes a brush's arithmetic sees besides the point itself.
+	float leaf_extent = 0.0f;
 };
 
 // Validates stack discipline and opcodes the way SDFInterpreter::Eval would
 // (tangerine/sdf_evaluator.cpp:1397-1600) and re-encodes the program.
 // Returns false with a message on malformed input.
 bool decode_words(const uint32_t* words, size_t count, DecodedProgram& out, std::string& error);
+
+// Absolute rounding slack of the pruning decisions and the brick certificate
+// for points p with |p| <= pmax (see the definition in tg_program.cpp).
+float rounding_slack(const DecodedProgram& d, float pmax);
 
 // Algorithmic cost of one evaluation of a record (adds, muls, min/max and
 // selects count 1, an FMA 2, sqrt and divide 1 each).  Fixed table.

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "../../include/tangerine_gpu.h"
@@ -27,7 +28,7 @@ bool entry_levels(const DecodedProgram& d, std::vector<uint32_t>& order, std::ve
 	for (size_t e = 0; e < n; ++e)
 	{
 		const HostEntry& x = d.entries[e];
-		const uint32_t op = x.rec >> 24;
+		const uint32_t op = entry_op(x.rec);
 		if (op - OP_SPHERE <= OP_PLANE - OP_SPHERE)
 			h[e] = 0;
 		else if (op == OP_FLATE || op == OP_SCALE_FIELD)
@@ -58,6 +59,15 @@ int prune_cells(hipStream_t stream, const tg_program* p, const float origin[3], 
 
 	const uint64_t ncells = uint64_t(counts[0]) * counts[1] * counts[2];
 	if (ncells == 0 || ncells >= (1ull << 31)) return TG_OK;
+	// decision radius: the cell radius with the Lipschitz tolerance and the
+	// rounding slack for points of the cells' box
+	float pmax = 0.0f;
+	for (int i = 0; i < 3; ++i)
+	{
+		const float a = std::fabs(origin[i]), b = std::fabs(origin[i] + float(counts[i]) * cell[i]);
+		pmax += std::max(a, b) * std::max(a, b);
+	}
+	const float decide_radius = radius * (1.0f + 1.0e-5f) + rounding_slack(d, std::sqrt(pmax));
 	// Fewer cells per wave only when there are very few cells (every wave
 	// gathers the whole program, so smaller groups cost L2 traffic).
 	uint32_t cpw = 64;
@@ -99,7 +109,7 @@ int prune_cells(hipStream_t stream, const tg_program* p, const float origin[3], 
 		pa.n_binary = d.binary_entries;
 		pa.ox = origin[0]; pa.oy = origin[1]; pa.oz = origin[2];
 		pa.cx = cell[0]; pa.cy = cell[1]; pa.cz = cell[2];
-		pa.radius = radius;
+		pa.radius = decide_radius;
 		pa.nx = counts[0]; pa.ny = counts[1]; pa.nz = counts[2];
 		pa.slots = whole_slots;
 		pa.whole_flops = uint32_t(d.flops);
@@ -118,7 +128,7 @@ int prune_cells(hipStream_t stream, const tg_program* p, const float origin[3], 
 			la.cells_per_wave = cpw;
 			la.ox = pa.ox; la.oy = pa.oy; la.oz = pa.oz;
 			la.cx = pa.cx; la.cy = pa.cy; la.cz = pa.cz;
-			la.radius = radius;
+			la.radius = decide_radius;
 			la.nx = pa.nx; la.ny = pa.ny; la.nz = pa.nz;
 			la.values = out.values.as<float>();
 			la.scratch = pa.scratch;

@@ -195,6 +195,7 @@ def _info_dict(info) -> dict:
         "set_ops": int(info.set_ops),
         "lipschitz": bool(info.lipschitz),
         "flops_per_eval": float(info.flops_per_eval),
+        "leaf_extent": float(info.leaf_extent),
     }
 
 
