@@ -423,6 +423,9 @@ int tg_batch_replay(tg_batch* batch, double* eval_ms);
  * shape among seven; eval_ms is the best whole-batch time seen. */
 int tg_batch_autotune(tg_batch* batch, tg_batch_tune_result* out);
 int tg_batch_set_tuning(tg_batch* batch, uint32_t points_per_lane, uint32_t waves, uint32_t workgroups_per_cu);
+/* Tree `index`'s mesher, owned by the batch: read its results (counts, mesh,
+ * grid, stats, bricks, STL/PLY writers); exports, tuning and destroy on it
+ * return TG_ERR_INVALID (its buffers are captured in the batch's graph). */
 tg_mesher* tg_batch_mesher(tg_batch* batch, uint32_t index);
 int tg_batch_stats_get(const tg_batch* batch, tg_batch_stats* out);
 

@@ -57,6 +57,10 @@ struct tg_mesher
 	int last_waves = 0;
 	uint32_t last_max_len = 0, last_max_binary = 0, last_lds = 0;
 	float last_block_ms = 0.0f;   // the last k_eval_blocks launch (HIP events)
+	// Owned by a tg_batch (tg_batch_mesher): its buffers are captured in the
+	// batch's graph, so calls that would lay them out again, and destroy, are
+	// refused; the batch destroys it.
+	bool batch_owned = false;
 };
 
 namespace {
@@ -373,9 +377,18 @@ extern "C" int tg_mesher_create(tg_mesher** out)
 	return TG_OK;
 }
 
-extern "C" void tg_mesher_destroy(tg_mesher* m)
+namespace {
+
+// Refuses calls that re-lay out a batch-owned mesher's buffers.
+int refuse_batch_owned(const tg_mesher* m, const char* what)
 {
-	if (!m) return;
+	if (m && m->batch_owned)
+		return fail(TG_ERR_INVALID, std::string(what) + ": the mesher belongs to a tg_batch (read its results only)");
+	return TG_OK;
+}
+
+void destroy_mesher(tg_mesher* m)
+{
 	if (m->stream) (void)hipStreamSynchronize(m->stream);
 	for (DevBuf* b : { &m->class_list, &m->pool, &m->signs, &m->active, &m->word_count, &m->word_base, &m->quad_count, &m->quad_base,
 	                   &m->vertices, &m->triangles, &m->scan_tmp, &m->dense, &m->counters, &m->brick_flags,
@@ -387,6 +400,14 @@ extern "C" void tg_mesher_destroy(tg_mesher* m)
 		if (e) (void)hipEventDestroy(e);
 	if (m->stream) (void)hipStreamDestroy(m->stream);
 	delete m;
+}
+
+}  // namespace
+
+extern "C" void tg_mesher_destroy(tg_mesher* m)
+{
+	if (!m || refuse_batch_owned(m, "tg_mesher_destroy")) return;
+	destroy_mesher(m);
 }
 
 namespace {
@@ -606,6 +627,7 @@ const float* tg::mesher_vertices(const tg_mesher* m) { return m->vertices.as<con
 extern "C" int tg_mesher_export(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t flags)
 {
 	if (!m || !p) return fail(TG_ERR_INVALID, "tg_mesher_export: null argument");
+	if (int rc = refuse_batch_owned(m, "tg_mesher_export")) return rc;
 	return run_export(m, p, g, flags, false, nullptr);
 }
 
@@ -613,6 +635,7 @@ extern "C" int tg_mesher_export_culled(tg_mesher* m, const tg_program* p, const 
 	uint32_t flags)
 {
 	if (!m || !p) return fail(TG_ERR_INVALID, "tg_mesher_export_culled: null argument");
+	if (int rc = refuse_batch_owned(m, "tg_mesher_export_culled")) return rc;
 	return run_export(m, p, g, flags, true, s);
 }
 
@@ -652,6 +675,7 @@ extern "C" int tg_mesher_set_tuning(tg_mesher* m, uint32_t points_per_lane, uint
 	if (!m || !(points_per_lane == 2 || points_per_lane == 4 || points_per_lane == 8) ||
 		!(waves == 1 || waves == 2 || waves == 4))
 		return fail(TG_ERR_INVALID, "tg_mesher_set_tuning: points per lane in {2,4,8}, waves in {1,2,4}");
+	if (int rc = refuse_batch_owned(m, "tg_mesher_set_tuning")) return rc;
 	m->points_per_lane = int(points_per_lane);
 	m->waves = int(waves);
 	m->min_prune_len = min_prune_len;
@@ -662,6 +686,7 @@ extern "C" int tg_mesher_autotune(tg_mesher* m, const tg_program* p, const tg_gr
 	tg_tune_result* out)
 {
 	if (!m || !p) return fail(TG_ERR_INVALID, "tg_mesher_autotune: null argument");
+	if (int rc = refuse_batch_owned(m, "tg_mesher_autotune")) return rc;
 	int rc = validate_grid(g);
 	if (rc) return rc;
 	if ((flags & TG_FLAG_NO_PRUNE) || !p->decoded.lipschitz)
@@ -743,6 +768,7 @@ extern "C" int tg_mesher_autotune(tg_mesher* m, const tg_program* p, const tg_gr
 extern "C" int tg_mesher_mesh_values(tg_mesher* m, const float* values, const tg_grid* g)
 {
 	if (!m || !values) return fail(TG_ERR_INVALID, "tg_mesher_mesh_values: null argument");
+	if (int rc = refuse_batch_owned(m, "tg_mesher_mesh_values")) return rc;
 	int rc = validate_grid(g);
 	if (rc) return rc;
 	m->have_grid = false;
@@ -1054,6 +1080,7 @@ extern "C" int tg_mesher_export_sodapop(tg_mesher* m, const tg_program* p, const
 	int shuffle, uint32_t flags)
 {
 	if (!m || !p || !bounds) return fail(TG_ERR_INVALID, "tg_mesher_export_sodapop: null argument");
+	if (int rc = refuse_batch_owned(m, "tg_mesher_export_sodapop")) return rc;
 	if (!(density >= 1.0f)) return fail(TG_ERR_INVALID, "tg_mesher_export_sodapop: density must be >= 1");
 	tg_grid g{};
 	int rc = sodapop_grid(bounds, density, &g);
@@ -1108,6 +1135,7 @@ extern "C" int tg_mesher_export_points(tg_mesher* m, const tg_program* p, const 
 	const float step[3], int refine_iterations, uint32_t flags)
 {
 	if (!m || !p || !lo || !hi || !step) return fail(TG_ERR_INVALID, "tg_mesher_export_points: null argument");
+	if (int rc = refuse_batch_owned(m, "tg_mesher_export_points")) return rc;
 	if (refine_iterations < 0) return fail(TG_ERR_INVALID, "tg_mesher_export_points: negative refine iterations");
 	CloudArgs a{};
 	uint64_t n[3];
@@ -1300,6 +1328,7 @@ extern "C" int tg_mesher_export_vox(tg_mesher* m, const tg_program* p, const flo
 	int color_index, const char* path, uint32_t flags, uint64_t* voxel_count)
 {
 	if (!m || !p || !bounds) return fail(TG_ERR_INVALID, "tg_mesher_export_vox: null argument");
+	if (int rc = refuse_batch_owned(m, "tg_mesher_export_vox")) return rc;
 	VoxArgs a{};
 	for (int i = 0; i < 3; ++i)
 	{
@@ -1609,7 +1638,7 @@ extern "C" void tg_batch_destroy(tg_batch* b)
 	if (!b) return;
 	if (b->stream) (void)hipStreamSynchronize(b->stream);
 	drop_graph(b);
-	for (tg_mesher* m : b->meshers) tg_mesher_destroy(m);
+	for (tg_mesher* m : b->meshers) destroy_mesher(m);
 	for (DevBuf* d : { &b->trees[0], &b->trees[1], &b->first[0], &b->first[1], &b->head, &b->work, &b->class_counts })
 		d->release();
 	for (hipEvent_t e : b->ev)
@@ -1651,11 +1680,12 @@ extern "C" int tg_batch_export(tg_batch* b, const tg_program* const* programs, c
 		tg_mesher* m = nullptr;
 		int rc = tg_mesher_create(&m);
 		if (rc) return rc;
+		m->batch_owned = true;
 		b->meshers.push_back(m);
 	}
 	while (b->meshers.size() > count)
 	{
-		tg_mesher_destroy(b->meshers.back());
+		destroy_mesher(b->meshers.back());
 		b->meshers.pop_back();
 	}
 	b->programs.assign(programs, programs + count);

@@ -86,3 +86,28 @@ def test_batch_rejects_bad_input(gpu):
     with pytest.raises(TangerineError, match="no batch"):
         b.replay()
     b.close()
+
+
+def test_batch_owned_meshers_refuse_relayout(gpu):
+    """A tree's mesher belongs to the batch: calls that would lay its buffers
+    out again (the captured graph points at them) and destroy are refused;
+    reads still work and the batch replays unchanged."""
+    import ctypes as C
+    from tangerine_amd import _lib, scenes
+    from tangerine_amd.batch import Batch
+    lib = _lib.load()
+    progs = [scenes.build_sdf(scenes.random_csg_spec(s, 16)).program() for s in (21, 22)]
+    grids = [scenes.grid_for(-6.0, 6.0, 40), scenes.grid_for(-6.0, 6.0, 24)]
+    b = Batch()
+    b.export(progs, grids)
+    before = bits(b.mesher(0).grid()).copy()
+    h = lib.tg_batch_mesher(b.h, 0)
+    g = _lib.grid_struct(*grids[1])
+    assert lib.tg_mesher_export(h, progs[1].handle, C.byref(g), 0) == -1
+    assert b"belongs to a tg_batch" in lib.tg_last_error()
+    assert lib.tg_mesher_set_tuning(h, 2, 4, 0) == -1
+    lib.tg_mesher_destroy(h)  # refused: the handle stays valid
+    assert b"belongs to a tg_batch" in lib.tg_last_error()
+    assert b.replay() > 0
+    assert np.array_equal(bits(b.mesher(0).grid()), before)
+    b.close()
