@@ -43,7 +43,6 @@ struct tg_mesher
 	bool have_normals = false;
 	DevBuf counters;
 	DevBuf brick_flags, brick_active, tile_mask, block_list, tile_list;
-	DevBuf class_list;            // batch: this tree's blocks split by program length (short | long)
 	uint64_t nbricks = 0;         // bricks of the last culled export (0: none)
 	CellPrograms blocks;
 	MeshArgs args{};
@@ -390,7 +389,7 @@ int refuse_batch_owned(const tg_mesher* m, const char* what)
 void destroy_mesher(tg_mesher* m)
 {
 	if (m->stream) (void)hipStreamSynchronize(m->stream);
-	for (DevBuf* b : { &m->class_list, &m->pool, &m->signs, &m->active, &m->word_count, &m->word_base, &m->quad_count, &m->quad_base,
+	for (DevBuf* b : { &m->pool, &m->signs, &m->active, &m->word_count, &m->word_base, &m->quad_count, &m->quad_base,
 	                   &m->vertices, &m->triangles, &m->scan_tmp, &m->dense, &m->counters, &m->brick_flags,
 	                   &m->brick_active, &m->tile_mask, &m->block_list, &m->tile_list, &m->cubes, &m->normals,
 	                   &m->exchange })
@@ -1438,12 +1437,14 @@ extern "C" int tg_mesher_grid_get(const tg_mesher* m, tg_grid* out)
 // ---- batched export: many trees, one persistent launch -----------------------
 //
 // Every tree keeps its own mesher (grid layout, pruned block programs, sample
-// tiles, mesh).  The evaluation of all trees' blocks is one launch of
-// k_eval_batch over a device queue; that launch and the reset of the queue
-// head and work counters are captured in a hipGraph, replayed by
-// tg_batch_export and tg_batch_replay.  Trees are queued heaviest first
-// (pruned-program entries as the work estimate) so that the tail of the queue
-// holds small items.
+// tiles, mesh).  The evaluation of all trees' blocks is one persistent launch
+// of k_eval_batch over a device queue of (tree, block) items, most expensive
+// block program first (so that the tail of the queue holds short items); the
+// launch and the reset of the queue head and work counters are captured in a
+// hipGraph, replayed by tg_batch_export and tg_batch_replay.  Each workgroup
+// keeps its block's decision tables and tile programs in its own slice of
+// global scratch (LDS holds only the tile descriptors and the spill region),
+// so programs of any length run at the same occupancy.
 
 struct tg_batch
 {
@@ -1452,18 +1453,15 @@ struct tg_batch
 	std::vector<tg_mesher*> meshers;
 	std::vector<const tg_program*> programs;
 	std::vector<tg_grid> grids;
-	// Two queues, each one persistent launch: blocks whose programs are short
-	// (at most kShortProgram entries, most of the work) run with an LDS layout
-	// sized for them, the long ones with one sized for the longest program.
-	DevBuf trees[2], first[2], head, work, class_counts;
+	DevBuf trees, items, head, work;
+	DevBuf tables;                // per-workgroup decision tables and tile pools
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
-	BatchArgs args[2] = {};
+	BatchArgs args = {};
 	bool fast = false;
-	int points_per_lane = 4;      // short-program launch (tg_batch_set_tuning / autotune)
+	int points_per_lane = 4;      // launch shape (tg_batch_set_tuning / autotune)
 	int waves = 4;
-	int workgroups_per_cu = 2;
-	int long_points = 8, long_waves = 4, long_wgs = 1;  // long-program launch (autotune)
+	int workgroups_per_cu = 4;
 	int cus = 0;
 	int last_points_per_lane = 0, last_waves = 0, last_workgroups = 0;
 	tg_batch_stats stats{};
@@ -1479,111 +1477,45 @@ void drop_graph(tg_batch* b)
 	b->graph = nullptr;
 }
 
-constexpr uint32_t kShortProgram = 256;  // entries: the short-program queue's LDS tables
-
-// Thread per block of one tree: appends it to the short or the long list
-// (counts[0], counts[1]); blocks left with the whole program are long.
-__global__ void __launch_bounds__(256) k_split_blocks(const BlockDesc* desc, uint32_t n, uint32_t* short_list,
-	uint32_t* long_list, unsigned long long* counts)
+// Launch shape for (points, waves, workgroups per CU): the spill region gets
+// all the LDS the workgroups per CU leave beside the tile descriptors; each
+// workgroup gets its slice of the global tables.
+int shape_batch(tg_batch* b, int points, int waves, int wgs_per_cu)
 {
-	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-	const bool valid = i < n;
-	bool is_short = false;
-	if (valid)
-	{
-		const BlockDesc bd = desc[i];
-		is_short = bd.offset != 0xffffffffu && bd.length <= kShortProgram;
-	}
-	const int lane = threadIdx.x & 63;
-	const uint64_t s_votes = __ballot(valid && is_short), l_votes = __ballot(valid && !is_short);
-	const uint64_t below = (1ull << lane) - 1ull;
-	unsigned long long s_base = 0, l_base = 0;
-	if (lane == 0)
-	{
-		if (s_votes) s_base = atomicAdd(&counts[0], (unsigned long long)__popcll(s_votes));
-		if (l_votes) l_base = atomicAdd(&counts[1], (unsigned long long)__popcll(l_votes));
-	}
-	s_base = __shfl(s_base, 0);
-	l_base = __shfl(l_base, 0);
-	if (!valid) return;
-	if (is_short)
-		short_list[s_base + __popcll(s_votes & below)] = i;
-	else
-		long_list[l_base + __popcll(l_votes & below)] = i;
-}
-
-struct LaunchShape
-{
-	int pts = 0, waves = 0;
-	uint32_t wgs = 0;
-};
-
-// LDS layout and launch shape of one queue for (points, waves, workgroups per CU).
-int shape_queue(tg_batch* b, BatchArgs& ba, int points, int waves_in, int wgs_per_cu, LaunchShape& out)
-{
-	int pts = points, waves = waves_in;
-	auto lds_for = [&](int p, int w) {
-		return block_lds_bytes(block_spill_slots(ba.max_depth, p), p, w, ba.max_binary, ba.max_len);
-	};
-	while (pts > 2 && lds_for(pts, waves) > kMaxLds) pts >>= 1;
-	while (waves > 1 && lds_for(pts, waves) > kMaxLds) waves >>= 1;
-	ba.slots = block_spill_slots(ba.max_depth, pts);
+	BatchArgs& ba = b->args;
+	int pts = points;
+	const size_t per_wg = kMaxLds / size_t(std::max(1, wgs_per_cu));
+	while (pts > 2 && block_lds_bytes_global(block_spill_slots(ba.max_depth, pts), pts, waves) > per_wg) pts >>= 1;
+	if (block_lds_bytes_global(block_spill_slots(ba.max_depth, pts), pts, waves) > per_wg)
+		return fail(TG_ERR_LIMIT, "batch: spill region too large for the workgroups per CU");
+	ba.slots = block_spill_slots(ba.max_depth, pts, (per_wg - kTileInfoBytes) / size_t(waves));
 	ba.pool_words = 0;
-	{
-		// A persistent launch has a fixed number of workgroups per CU: the
-		// LDS they leave goes first to the spill region (deep tiles then keep
-		// all their points per lane), then to the tile pool, up to what the
-		// 64 tile programs of the longest block can need (a wave emitting its
-		// own tile programs, one walk of the block program per tile, is the
-		// slow path when they do not fit).
-		const size_t per_wg = kMaxLds / size_t(std::max(1, wgs_per_cu));
-		const size_t lists = size_t(waves) * block_list_bytes(ba.max_len);
-		const size_t fixed = block_table_bytes(ba.max_binary, ba.max_len) + lists;
-		if (per_wg > fixed)
-		{
-			const uint32_t roomy = block_spill_slots(ba.max_depth, pts, (per_wg - fixed) / size_t(waves));
-			if (block_lds_bytes(roomy, pts, waves, ba.max_binary, ba.max_len) <= per_wg) ba.slots = std::max(ba.slots, roomy);
-			const size_t used = block_lds_bytes(ba.slots, pts, waves, ba.max_binary, ba.max_len);
-			const size_t want = size_t(kTilesPerBlock) * block_list_bytes(ba.max_len);
-			const size_t extra = per_wg > used ? std::min(want > lists ? want - lists : size_t(0),
-			                                              (per_wg - used) & ~size_t(15)) : size_t(0);
-			ba.pool_words = uint32_t((lists + extra) / 4);
-		}
-	}
-	if (block_lds_bytes(ba.slots, pts, waves, ba.max_binary, ba.max_len, ba.pool_words) > kMaxLds)
-		return fail(TG_ERR_LIMIT, "batch: block programs too long for the LDS");
-	out.pts = pts;
-	out.waves = waves;
-	out.wgs = uint32_t(std::max(1, b->cus * wgs_per_cu));
+	ba.table_words = block_global_words(ba.max_binary, ba.max_len);
+	const uint32_t wgs = uint32_t(std::max(1, b->cus * wgs_per_cu));
+	int rc = b->tables.ensure(size_t(wgs) * ba.table_words * 4);
+	if (rc) return rc;
+	ba.tables = b->tables.as<uint32_t>();
+	b->last_points_per_lane = pts;
+	b->last_waves = waves;
+	b->last_workgroups = int(wgs);
 	return TG_OK;
 }
 
-// Captures the queue resets and both persistent launches with the current tuning.
+// Captures the queue reset and the persistent launch with the current tuning.
 int capture_batch(tg_batch* b)
 {
 	drop_graph(b);
-	LaunchShape shape[2];
-	for (int c = 0; c < 2; ++c)
-	{
-		if (b->args[c].total == 0) continue;
-		const int rc = c == 0 ? shape_queue(b, b->args[0], b->points_per_lane, b->waves, b->workgroups_per_cu, shape[0])
-		                      : shape_queue(b, b->args[1], b->long_points, b->long_waves, b->long_wgs, shape[1]);
-		if (rc) return rc;
-	}
+	int rc = shape_batch(b, b->points_per_lane, b->waves, b->workgroups_per_cu);
+	if (rc) return rc;
 	const uint32_t nwork = uint32_t(2 * b->meshers.size());
 	TG_CHECK_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "capture begin");
-	hipError_t e = hipSuccess;
-	for (int c = 0; c < 2 && e == hipSuccess; ++c)
-		e = b->fast ? fast::launch_batch_reset(b->args[c].head, b->work.as<unsigned long long>(), c == 0 ? nwork : 0,
-		                                       b->stream)
-		            : exact::launch_batch_reset(b->args[c].head, b->work.as<unsigned long long>(), c == 0 ? nwork : 0,
-		                                        b->stream);
-	for (int c = 1; c >= 0 && e == hipSuccess; --c)
-	{
-		if (b->args[c].total == 0) continue;
-		e = b->fast ? fast::launch_batch(b->args[c], shape[c].pts, uint32_t(shape[c].waves), shape[c].wgs, b->stream)
-		            : exact::launch_batch(b->args[c], shape[c].pts, uint32_t(shape[c].waves), shape[c].wgs, b->stream);
-	}
+	hipError_t e = b->fast ? fast::launch_batch_reset(b->args.head, b->work.as<unsigned long long>(), nwork, b->stream)
+	                       : exact::launch_batch_reset(b->args.head, b->work.as<unsigned long long>(), nwork, b->stream);
+	if (e == hipSuccess)
+		e = b->fast ? fast::launch_batch(b->args, b->last_points_per_lane, uint32_t(b->last_waves),
+		                                 uint32_t(b->last_workgroups), b->stream)
+		            : exact::launch_batch(b->args, b->last_points_per_lane, uint32_t(b->last_waves),
+		                                  uint32_t(b->last_workgroups), b->stream);
 	hipGraph_t g = nullptr;
 	const hipError_t ee = hipStreamEndCapture(b->stream, &g);
 	if (e != hipSuccess)
@@ -1594,10 +1526,6 @@ int capture_batch(tg_batch* b)
 	if (ee != hipSuccess) return hip_fail(ee, "batch capture end");
 	b->graph = g;
 	TG_CHECK_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "batch graph instantiate");
-	const int main = b->args[0].total ? 0 : 1;
-	b->last_points_per_lane = shape[main].pts;
-	b->last_waves = shape[main].waves;
-	b->last_workgroups = int(shape[main].wgs);
 	return TG_OK;
 }
 
@@ -1639,7 +1567,7 @@ extern "C" void tg_batch_destroy(tg_batch* b)
 	if (b->stream) (void)hipStreamSynchronize(b->stream);
 	drop_graph(b);
 	for (tg_mesher* m : b->meshers) destroy_mesher(m);
-	for (DevBuf* d : { &b->trees[0], &b->trees[1], &b->first[0], &b->first[1], &b->head, &b->work, &b->class_counts })
+	for (DevBuf* d : { &b->trees, &b->items, &b->head, &b->work, &b->tables })
 		d->release();
 	for (hipEvent_t e : b->ev)
 		if (e) (void)hipEventDestroy(e);
@@ -1693,16 +1621,10 @@ extern "C" int tg_batch_export(tg_batch* b, const tg_program* const* programs, c
 	auto t0 = std::chrono::steady_clock::now();
 	int rc = b->work.ensure(size_t(count) * 16);
 	if (!rc) rc = b->head.ensure(256);
-	if (!rc) rc = b->class_counts.ensure(size_t(count) * 16);
-	for (int c = 0; c < 2 && !rc; ++c)
-	{
-		rc = b->trees[c].ensure(size_t(count) * sizeof(BlockEvalArgs));
-		if (!rc) rc = b->first[c].ensure(size_t(count + 1) * 4);
-	}
+	if (!rc) rc = b->trees.ensure(size_t(count) * sizeof(BlockEvalArgs));
 	if (rc) return rc;
 	// 1. layout and level-1 pruning per tree
 	std::vector<BlockEvalArgs> args(count);
-	std::vector<double> weight(count);
 	BatchArgs ba{};
 	uint64_t samples = 0;
 	for (uint32_t i = 0; i < count; ++i)
@@ -1715,88 +1637,48 @@ extern "C" int tg_batch_export(tg_batch* b, const tg_program* const* programs, c
 		if (rc) return rc;
 		if (!m->blocks.valid) return fail(TG_ERR_LIMIT, "tg_batch_export: a tree could not be pruned");
 		args[i] = block_args(m, programs[i], &grids[i], nullptr, nullptr, 0, b->work.as<unsigned long long>() + 2 * i);
-		weight[i] = double(m->blocks.entries) + double(m->blocks.ncells);
 		ba.max_depth = std::max(ba.max_depth, args[i].max_depth);
 		ba.max_binary = std::max(ba.max_binary, args[i].max_binary);
 		ba.max_len = std::max(ba.max_len, args[i].max_len);
 		samples += uint64_t(m->args.nx) * m->args.ny * m->args.nz;
 	}
-	// Trees whose tables would not fit the LDS beside the others run unpruned tiles.
-	while (block_lds_bytes(block_spill_slots(ba.max_depth, 2), 2, 1, ba.max_binary, ba.max_len) > kMaxLds)
+	// 2. the queue: every tree's blocks, the most expensive block program
+	// first (program cost x tiles of the block)
+	struct Item
 	{
-		uint32_t worst = 0;
-		for (uint32_t i = 1; i < count; ++i)
-			if (args[i].max_len > args[worst].max_len) worst = i;
-		args[worst].prune_tiles = 0;
-		args[worst].max_len = args[worst].max_binary = 0;
-		ba.max_binary = ba.max_len = 0;
-		for (uint32_t i = 0; i < count; ++i)
-		{
-			ba.max_binary = std::max(ba.max_binary, args[i].max_binary);
-			ba.max_len = std::max(ba.max_len, args[i].max_len);
-		}
-	}
-	// 2. every tree's blocks split by program length, one queue per class
-	TG_CHECK_HIP(hipMemsetAsync(b->class_counts.ptr, 0, size_t(count) * 16, b->stream), "count reset");
-	unsigned long long* ccounts = b->class_counts.as<unsigned long long>();
+		double cost;
+		uint32_t tree, block;
+	};
+	std::vector<Item> order;
+	std::vector<BlockDesc> desc;
 	for (uint32_t i = 0; i < count; ++i)
 	{
-		tg_mesher* m = b->meshers[i];
 		const uint32_t nb = args[i].nblocks;
-		rc = m->class_list.ensure(std::max<uint64_t>(nb, 1) * 8);
-		if (rc) return rc;
-		if (nb)
-		{
-			hipLaunchKernelGGL(k_split_blocks, dim3(grid_for(nb)), dim3(256), 0, b->stream, args[i].desc, nb,
-				m->class_list.as<uint32_t>(), m->class_list.as<uint32_t>() + nb, ccounts + 2 * i);
-			TG_CHECK_HIP(hipGetLastError(), "block split");
-		}
+		desc.resize(nb);
+		TG_CHECK_HIP(hipMemcpyAsync(desc.data(), args[i].desc, size_t(nb) * sizeof(BlockDesc), hipMemcpyDeviceToHost,
+			b->meshers[i]->stream), "block descriptors");
+		TG_CHECK_HIP(hipStreamSynchronize(b->meshers[i]->stream), "block descriptors");
+		for (uint32_t k = 0; k < nb; ++k) order.push_back(Item{ double(desc[k].flops), i, k });
 	}
-	std::vector<unsigned long long> split(2 * size_t(count));
-	TG_CHECK_HIP(hipMemcpyAsync(split.data(), ccounts, split.size() * 8, hipMemcpyDeviceToHost, b->stream), "split read");
-	TG_CHECK_HIP(hipStreamSynchronize(b->stream), "block split");
-	// queue order: heaviest trees first
-	std::vector<uint32_t> order(count);
-	for (uint32_t i = 0; i < count; ++i) order[i] = i;
-	std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return weight[x] > weight[y]; });
-	for (int c = 0; c < 2; ++c)
+	std::stable_sort(order.begin(), order.end(), [](const Item& x, const Item& y) { return x.cost > y.cost; });
+	std::vector<uint32_t> items(2 * order.size());
+	for (size_t k = 0; k < order.size(); ++k)
 	{
-		std::vector<BlockEvalArgs> queued;
-		std::vector<uint32_t> first(1, 0);
-		BatchArgs q{};
-		for (uint32_t k = 0; k < count; ++k)
-		{
-			const uint32_t i = order[k];
-			const uint32_t nb = uint32_t(split[2 * i + c]);
-			if (nb == 0) continue;
-			BlockEvalArgs x = args[i];
-			x.block_list = b->meshers[i]->class_list.as<const uint32_t>() + (c == 0 ? 0 : args[i].nblocks);
-			x.nblocks = nb;
-			if (c == 0)
-			{
-				// short programs: at most kShortProgram entries, half of them binary
-				x.max_len = std::min(x.max_len, kShortProgram);
-				x.max_binary = std::min(x.max_binary, kShortProgram / 2);
-			}
-			queued.push_back(x);
-			first.push_back(first.back() + nb);
-			q.max_depth = std::max(q.max_depth, x.max_depth);
-			q.max_binary = std::max(q.max_binary, x.max_binary);
-			q.max_len = std::max(q.max_len, x.max_len);
-		}
-		if (!queued.empty())
-		{
-			TG_CHECK_HIP(hipMemcpy(b->trees[c].ptr, queued.data(), queued.size() * sizeof(BlockEvalArgs),
-				hipMemcpyHostToDevice), "upload");
-			TG_CHECK_HIP(hipMemcpy(b->first[c].ptr, first.data(), first.size() * 4, hipMemcpyHostToDevice), "upload");
-		}
-		q.trees = b->trees[c].as<const BlockEvalArgs>();
-		q.first = b->first[c].as<const uint32_t>();
-		q.ntrees = uint32_t(queued.size());
-		q.total = first.back();
-		q.head = b->head.as<unsigned int>() + 16 * c;
-		b->args[c] = q;
+		items[2 * k] = order[k].tree;
+		items[2 * k + 1] = order[k].block;
 	}
+	rc = b->items.ensure(std::max<size_t>(items.size(), 2) * 4);
+	if (rc) return rc;
+	TG_CHECK_HIP(hipMemcpy(b->trees.ptr, args.data(), size_t(count) * sizeof(BlockEvalArgs), hipMemcpyHostToDevice),
+		"upload");
+	if (!items.empty())
+		TG_CHECK_HIP(hipMemcpy(b->items.ptr, items.data(), items.size() * 4, hipMemcpyHostToDevice), "upload");
+	ba.trees = b->trees.as<const BlockEvalArgs>();
+	ba.items = b->items.as<const uint32_t>();
+	ba.ntrees = count;
+	ba.total = uint32_t(order.size());
+	ba.head = b->head.as<unsigned int>();
+	b->args = ba;
 	auto t1 = std::chrono::steady_clock::now();
 	// 3. one persistent launch (captured), then surface nets per tree
 	rc = capture_batch(b);
@@ -1810,9 +1692,8 @@ extern "C" int tg_batch_export(tg_batch* b, const tg_program* const* programs, c
 	double mesh_ms = 0;
 	uint64_t evals = 0;
 	double flops = 0;
-	for (uint32_t k = 0; k < count; ++k)
+	for (uint32_t i = 0; i < count; ++i)
 	{
-		const uint32_t i = order[k];
 		tg_mesher* m = b->meshers[i];
 		rc = surface_nets(m);
 		if (rc) return rc;
@@ -1836,16 +1717,16 @@ extern "C" int tg_batch_export(tg_batch* b, const tg_program* const* programs, c
 	st.mesh_ms = mesh_ms;
 	st.total_ms = std::chrono::duration<double, std::milli>(t3 - t0).count();
 	st.trees = count;
-	st.blocks = b->args[0].total + b->args[1].total;
+	st.blocks = b->args.total;
 	st.samples = samples;
 	st.exact_evals = evals;
 	st.flops = flops;
 	st.points_per_lane = uint32_t(b->last_points_per_lane);
 	st.waves = uint32_t(b->last_waves);
 	st.workgroups = uint32_t(b->last_workgroups);
-	const BatchArgs& q = b->args[b->args[0].total ? 0 : 1];
-	st.lds_bytes = uint32_t(block_lds_bytes(q.slots, std::max(2, b->last_points_per_lane), std::max(1, b->last_waves),
-		q.max_binary, q.max_len, q.pool_words));
+	const BatchArgs& q = b->args;
+	st.lds_bytes = uint32_t(block_lds_bytes_global(q.slots, std::max(2, b->last_points_per_lane),
+		std::max(1, b->last_waves)));
 	st.max_len = q.max_len;
 	st.max_binary = q.max_binary;
 	st.max_depth = q.max_depth;
@@ -1867,59 +1748,18 @@ extern "C" int tg_batch_autotune(tg_batch* b, tg_batch_tune_result* out)
 {
 	if (!b) return fail(TG_ERR_INVALID, "tg_batch_autotune: null argument");
 	if (!b->exec) return fail(TG_ERR_INVALID, "tg_batch_autotune: export a batch first");
+	// points per lane x (waves, workgroups per CU): 18 shapes
 	const int pts_c[3] = { 2, 4, 8 };
-	const int waves_c[2] = { 2, 4 };
-	const int wg_c[3] = { 1, 2, 4 };
+	const int shape_c[6][2] = { { 4, 2 }, { 4, 3 }, { 4, 4 }, { 4, 6 }, { 2, 6 }, { 2, 8 } };
 	tg_batch_tune_result r{};
 	double best = 1e300;
 	uint32_t k = 0;
 	for (int pi = 0; pi < 3; ++pi)
-		for (int wi = 0; wi < 2; ++wi)
-			for (int gi = 0; gi < 3; ++gi)
-			{
-				b->points_per_lane = pts_c[pi];
-				b->waves = waves_c[wi];
-				b->workgroups_per_cu = wg_c[gi];
-				int rc = capture_batch(b);
-				if (rc) return rc;
-				float t = 0, sum = 0;
-				for (int rep = 0; rep < 3; ++rep)
-				{
-					rc = replay_batch(b, &t);
-					if (rc) return rc;
-					if (rep > 0) sum += t;
-				}
-				const double ms = 0.5 * double(sum);
-				r.candidate_points[k] = uint32_t(b->last_points_per_lane);
-				r.candidate_waves[k] = uint32_t(b->last_waves);
-				r.candidate_workgroups_per_cu[k] = uint32_t(wg_c[gi]);
-				r.candidate_ms[k] = ms;
-				if (ms < best)
-				{
-					best = ms;
-					r.points_per_lane = uint32_t(b->last_points_per_lane);
-					r.waves = uint32_t(b->last_waves);
-					r.workgroups_per_cu = uint32_t(wg_c[gi]);
-				}
-				++k;
-			}
-	r.candidates = k;
-	r.eval_ms = best;
-	b->points_per_lane = int(r.points_per_lane);
-	b->waves = int(r.waves);
-	b->workgroups_per_cu = int(r.workgroups_per_cu);
-	if (b->args[1].total)
-	{
-		// the long-program queue: a few shapes, the short queue's choice fixed
-		const int long_c[7][3] = { { 8, 4, 1 }, { 4, 4, 1 }, { 8, 2, 1 }, { 4, 2, 2 }, { 2, 4, 2 }, { 4, 4, 2 },
-		                           { 2, 4, 1 } };
-		double best_long = 1e300;
-		int pick = 0;
-		for (int c = 0; c < 7; ++c)
+		for (int si = 0; si < 6; ++si)
 		{
-			b->long_points = long_c[c][0];
-			b->long_waves = long_c[c][1];
-			b->long_wgs = long_c[c][2];
+			b->points_per_lane = pts_c[pi];
+			b->waves = shape_c[si][0];
+			b->workgroups_per_cu = shape_c[si][1];
 			int rc = capture_batch(b);
 			if (rc) return rc;
 			float t = 0, sum = 0;
@@ -1929,17 +1769,25 @@ extern "C" int tg_batch_autotune(tg_batch* b, tg_batch_tune_result* out)
 				if (rc) return rc;
 				if (rep > 0) sum += t;
 			}
-			if (0.5 * double(sum) < best_long)
+			const double ms = 0.5 * double(sum);
+			r.candidate_points[k] = uint32_t(b->last_points_per_lane);
+			r.candidate_waves[k] = uint32_t(b->last_waves);
+			r.candidate_workgroups_per_cu[k] = uint32_t(shape_c[si][1]);
+			r.candidate_ms[k] = ms;
+			if (ms < best)
 			{
-				best_long = 0.5 * double(sum);
-				pick = c;
+				best = ms;
+				r.points_per_lane = uint32_t(b->last_points_per_lane);
+				r.waves = uint32_t(b->last_waves);
+				r.workgroups_per_cu = uint32_t(shape_c[si][1]);
 			}
+			++k;
 		}
-		b->long_points = long_c[pick][0];
-		b->long_waves = long_c[pick][1];
-		b->long_wgs = long_c[pick][2];
-		r.eval_ms = std::min(r.eval_ms, best_long);
-	}
+	r.candidates = k;
+	r.eval_ms = best;
+	b->points_per_lane = int(r.points_per_lane);
+	b->waves = int(r.waves);
+	b->workgroups_per_cu = int(r.workgroups_per_cu);
 	int rc = capture_batch(b);
 	if (rc) return rc;
 	if (out) *out = r;

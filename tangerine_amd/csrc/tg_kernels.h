@@ -161,13 +161,16 @@ struct BlockEvalArgs
 struct BatchArgs
 {
 	const BlockEvalArgs* trees;  // per tree (device array)
-	const uint32_t* first;       // [ntrees + 1] first queue item of each tree
+	const uint32_t* items;       // [2 * total]: tree, block of each queue item, in queue order
 	uint32_t ntrees;
 	uint32_t total;              // queue items
 	unsigned int* head;          // queue head (reset to 0 before each launch)
-	uint32_t slots, max_binary, max_len;  // LDS layout: the largest over the trees
+	uint32_t slots, max_binary, max_len;  // layout: the largest over the trees
 	uint32_t max_depth;                   // deepest block program over the trees (host: sizes `slots`)
-	uint32_t pool_words;                  // tile pool (block_pool_words; 0: the waves' lists only)
+	uint32_t pool_words;                  // tile pool in LDS (block_pool_words; 0: the waves' lists only)
+	uint32_t* tables;                     // non-null: per-workgroup decision tables and tile pool in global
+	                                      // memory, table_words each (block_global_words)
+	uint64_t table_words;
 };
 
 // Exact brick certificate: wave per block (4^3 cube bricks of 8^3 cubes,
@@ -373,6 +376,17 @@ __host__ __device__ inline size_t block_lds_bytes(uint32_t slots, int p, int wav
 {
 	return block_table_bytes(max_binary, max_len) + size_t(block_pool_words(pool_words, waves, max_len)) * 4 +
 	       size_t(waves) * spill_bytes_per_wave(slots, p);
+}
+// Blocks with their decision tables and tile pool in global memory: words of
+// one workgroup's slice (masks, liveness, then 64 tile programs of at most
+// max_len entries each), and the LDS that remains (tile descriptors, spill).
+__host__ __device__ inline uint64_t block_global_words(uint32_t max_binary, uint32_t max_len)
+{
+	return 4 * uint64_t(max_binary) + 2 * uint64_t(max_len) + uint64_t(kTilesPerBlock) * max_len + 64;
+}
+__host__ __device__ inline size_t block_lds_bytes_global(uint32_t slots, int p, int waves)
+{
+	return kTileInfoBytes + size_t(waves) * spill_bytes_per_wave(slots, p);
 }
 // Cell pruning: per-lane position stacks and the spill region (the decision
 // masks and liveness live in global scratch, prune_scratch_words per wave).
