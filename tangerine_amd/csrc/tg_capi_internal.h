@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <string>
+#include <vector>
 
 #include "tg_program.h"
 
@@ -37,6 +38,19 @@ struct tg_program
 namespace tg {
 int upload_program(const DecodedProgram& d, tg_program** out);
 
+// Progress of a cell pruning between prune_cells_start and prune_cells_finish.
+struct PruneState
+{
+	bool pending = false;        // a pass was issued; its counters are not read yet
+	bool level_parallel = false;
+	uint64_t ncells = 0, waves = 0, capacity = 0;
+	uint32_t cells_per_wave = 0;
+	float decide_radius = 0.0f;
+	size_t heights = 0;
+	std::vector<uint32_t> level_first;  // first entry position of every height
+	std::vector<uint32_t> level_order;  // entry positions by height (the upload's source, kept until finish)
+};
+
 // Level-1 pruned programs of a cell grid (BlockDesc per cell + entry pool).
 struct CellPrograms
 {
@@ -49,8 +63,12 @@ struct CellPrograms
 	uint32_t max_len = 0;     // most entries of a packed cell program
 	uint64_t whole_cells = 0; // cells left with the whole program (pool overflow)
 	bool valid = false;       // false: the program is not prunable here
+	PruneState state;
+	unsigned long long* host = nullptr;  // pinned copy of the pass counters
 	void release()
 	{
+		if (host) (void)hipHostFree(host);
+		host = nullptr;
 		pool.release();
 		desc.release();
 		counters.release();
@@ -62,6 +80,14 @@ struct CellPrograms
 };
 
 int prune_cells(hipStream_t stream, const tg_program* p, const float origin[3], const float cell[3], float radius,
+	const uint32_t counts[3], bool fast, CellPrograms& out);
+// The same in two halves: start issues the passes on `stream` without waiting;
+// finish waits for the stream, reruns the emission if the pool overflowed and
+// fills the counts (same arguments).  Between them the caller may issue other
+// work (other trees' pruning on other streams).
+int prune_cells_start(hipStream_t stream, const tg_program* p, const float origin[3], const float cell[3],
+	float radius, const uint32_t counts[3], bool fast, CellPrograms& out);
+int prune_cells_finish(hipStream_t stream, const tg_program* p, const float origin[3], const float cell[3],
 	const uint32_t counts[3], bool fast, CellPrograms& out);
 }
 

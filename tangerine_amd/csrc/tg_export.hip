@@ -47,6 +47,7 @@ struct tg_mesher
 	CellPrograms blocks;
 	MeshArgs args{};
 	uint64_t nverts = 0, ntris = 0;
+	uint32_t* tail = nullptr;     // pinned: vertex and quad totals read back by surface nets
 	bool have_grid = false;
 	tg_export_stats stats{};
 	int points_per_lane = 4;      // block kernel: points per lane (2, 4, 8; tg_mesher_autotune)
@@ -107,8 +108,11 @@ int layout(tg_mesher* m, const tg_grid* g)
 // Surface nets over the samples already in pool/signs: classify cube rows,
 // scan the per-word counts into vertex ids, list the active cubes, then one
 // thread per active cube places its vertex and counts and (after a scan of the
-// counts) writes its quads.
-int surface_nets(tg_mesher* m)
+// counts) writes its quads.  Three stages on the mesher's stream; each of the
+// first two ends with an asynchronous copy of two counts into the pinned
+// m->tail, which the next stage reads after the caller has synchronised the
+// stream (the batch runs each stage over all its trees before waiting).
+int surface_nets_classify(tg_mesher* m)
 {
 	MeshArgs& a = m->args;
 	const uint64_t words = uint64_t(a.sy) * a.sz * a.aw;
@@ -123,18 +127,21 @@ int surface_nets(tg_mesher* m)
 	TG_CHECK_HIP(exclusive_scan_u32(nullptr, nullptr, words, nullptr, tmp, m->stream), "scan sizing");
 	rc = m->scan_tmp.ensure(tmp);
 	if (rc) return rc;
-
 	TG_CHECK_HIP(mesh_classify(a, m->stream), "classify launch");
 	tmp = m->scan_tmp.bytes;
 	TG_CHECK_HIP(exclusive_scan_u32(a.word_count, a.word_base, words, m->scan_tmp.ptr, tmp, m->stream), "vertex scan");
-	uint32_t tail[2] = { 0, 0 };
-	TG_CHECK_HIP(hipMemcpyAsync(&tail[0], a.word_base + words - 1, 4, hipMemcpyDeviceToHost, m->stream), "count copy");
-	TG_CHECK_HIP(hipMemcpyAsync(&tail[1], a.word_count + words - 1, 4, hipMemcpyDeviceToHost, m->stream), "count copy");
-	TG_CHECK_HIP(hipStreamSynchronize(m->stream), "vertex count");
-	m->nverts = uint64_t(tail[0]) + tail[1];
+	TG_CHECK_HIP(hipMemcpyAsync(&m->tail[0], a.word_base + words - 1, 4, hipMemcpyDeviceToHost, m->stream), "count copy");
+	TG_CHECK_HIP(hipMemcpyAsync(&m->tail[1], a.word_count + words - 1, 4, hipMemcpyDeviceToHost, m->stream), "count copy");
+	return TG_OK;
+}
+
+int surface_nets_vertices(tg_mesher* m)
+{
+	MeshArgs& a = m->args;
+	m->nverts = uint64_t(m->tail[0]) + m->tail[1];
 	const uint64_t nv = m->nverts;
 	if (nv >= (1ull << 31)) return fail(TG_ERR_LIMIT, "surface nets: more than 2^31 vertices");
-	rc = m->vertices.ensure(std::max<uint64_t>(nv, 1) * 12);
+	int rc = m->vertices.ensure(std::max<uint64_t>(nv, 1) * 12);
 	if (!rc) rc = m->cubes.ensure(std::max<uint64_t>(nv, 1) * 4);
 	if (!rc) rc = m->quad_count.ensure(std::max<uint64_t>(nv, 1) * 4);
 	if (!rc) rc = m->quad_base.ensure(std::max<uint64_t>(nv, 1) * 4);
@@ -143,28 +150,42 @@ int surface_nets(tg_mesher* m)
 	a.cubes = m->cubes.as<uint32_t>();
 	a.cube_quads = m->quad_count.as<uint32_t>();
 	a.cube_qbase = m->quad_base.as<uint32_t>();
-	uint64_t quads = 0;
-	if (nv)
-	{
-		TG_CHECK_HIP(mesh_list_cubes(a, m->stream), "cube list launch");
-		TG_CHECK_HIP(mesh_vertices(a, nv, m->stream), "vertex launch");  // also each cube's quad count
-		tmp = 0;
-		TG_CHECK_HIP(exclusive_scan_u32(nullptr, nullptr, nv, nullptr, tmp, m->stream), "scan sizing");
-		rc = m->scan_tmp.ensure(tmp);
-		if (rc) return rc;
-		tmp = m->scan_tmp.bytes;
-		TG_CHECK_HIP(exclusive_scan_u32(a.cube_quads, a.cube_qbase, nv, m->scan_tmp.ptr, tmp, m->stream), "quad scan");
-		TG_CHECK_HIP(hipMemcpyAsync(&tail[0], a.cube_qbase + nv - 1, 4, hipMemcpyDeviceToHost, m->stream), "count copy");
-		TG_CHECK_HIP(hipMemcpyAsync(&tail[1], a.cube_quads + nv - 1, 4, hipMemcpyDeviceToHost, m->stream), "count copy");
-		TG_CHECK_HIP(hipStreamSynchronize(m->stream), "quad count");
-		quads = uint64_t(tail[0]) + tail[1];
-	}
+	m->tail[2] = m->tail[3] = 0;
+	if (!nv) return TG_OK;
+	TG_CHECK_HIP(mesh_list_cubes(a, m->stream), "cube list launch");
+	TG_CHECK_HIP(mesh_vertices(a, nv, m->stream), "vertex launch");  // also each cube's quad count
+	size_t tmp = 0;
+	TG_CHECK_HIP(exclusive_scan_u32(nullptr, nullptr, nv, nullptr, tmp, m->stream), "scan sizing");
+	rc = m->scan_tmp.ensure(tmp);
+	if (rc) return rc;
+	tmp = m->scan_tmp.bytes;
+	TG_CHECK_HIP(exclusive_scan_u32(a.cube_quads, a.cube_qbase, nv, m->scan_tmp.ptr, tmp, m->stream), "quad scan");
+	TG_CHECK_HIP(hipMemcpyAsync(&m->tail[2], a.cube_qbase + nv - 1, 4, hipMemcpyDeviceToHost, m->stream), "count copy");
+	TG_CHECK_HIP(hipMemcpyAsync(&m->tail[3], a.cube_quads + nv - 1, 4, hipMemcpyDeviceToHost, m->stream), "count copy");
+	return TG_OK;
+}
+
+int surface_nets_quads(tg_mesher* m)
+{
+	MeshArgs& a = m->args;
+	const uint64_t quads = uint64_t(m->tail[2]) + m->tail[3];
 	m->ntris = quads * 2;
-	rc = m->triangles.ensure(std::max<uint64_t>(quads, 1) * 24);
+	int rc = m->triangles.ensure(std::max<uint64_t>(quads, 1) * 24);
 	if (rc) return rc;
 	a.triangles = m->triangles.as<uint32_t>();
-	if (quads) TG_CHECK_HIP(mesh_quad_write(a, nv, m->stream), "quad write launch");
+	if (quads) TG_CHECK_HIP(mesh_quad_write(a, m->nverts, m->stream), "quad write launch");
 	return TG_OK;
+}
+
+int surface_nets(tg_mesher* m)
+{
+	int rc = surface_nets_classify(m);
+	if (rc) return rc;
+	TG_CHECK_HIP(hipStreamSynchronize(m->stream), "vertex count");
+	rc = surface_nets_vertices(m);
+	if (rc) return rc;
+	TG_CHECK_HIP(hipStreamSynchronize(m->stream), "quad count");
+	return surface_nets_quads(m);
 }
 
 // Dynamic LDS a workgroup may ask for: the CU's 160 KiB less room for the
@@ -249,16 +270,46 @@ unsigned grid_for(uint64_t n) { return unsigned((n + 255) / 256); }
 // Level-1 pruning: the program pruned for every 32^3-sample block (cell grid
 // with origin g - d/2, cell 32 d; the samples lie within 15.5 |d| of a block
 // centre).  m->blocks.valid is false when the program is not prunable.
-int prune_blocks(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t flags)
+// prune_blocks_start / prune_blocks_finish are its two halves (see
+// prune_cells_start).
+struct BlockGrid
+{
+	uint32_t counts[3];
+	float origin[3], cell[3], radius;
+};
+
+BlockGrid block_grid(const tg_mesher* m, const tg_grid* g)
 {
 	const MeshArgs& ma = m->args;
-	const uint32_t counts[3] = { (ma.tx + kBlockTiles - 1) / kBlockTiles, (ma.ty + kBlockTiles - 1) / kBlockTiles,
-	                             (ma.tz + kBlockTiles - 1) / kBlockTiles };
-	const float origin[3] = { g->x - 0.5f * g->dx, g->y - 0.5f * g->dy, g->z - 0.5f * g->dz };
-	const float cell[3] = { float(kBlockSamples) * g->dx, float(kBlockSamples) * g->dy, float(kBlockSamples) * g->dz };
+	BlockGrid b{};
+	b.counts[0] = (ma.tx + kBlockTiles - 1) / kBlockTiles;
+	b.counts[1] = (ma.ty + kBlockTiles - 1) / kBlockTiles;
+	b.counts[2] = (ma.tz + kBlockTiles - 1) / kBlockTiles;
+	b.origin[0] = g->x - 0.5f * g->dx; b.origin[1] = g->y - 0.5f * g->dy; b.origin[2] = g->z - 0.5f * g->dz;
+	b.cell[0] = float(kBlockSamples) * g->dx; b.cell[1] = float(kBlockSamples) * g->dy;
+	b.cell[2] = float(kBlockSamples) * g->dz;
 	const float half = 0.5f * float(kBlockSamples - 1);
-	const float radius = half * std::sqrt(g->dx * g->dx + g->dy * g->dy + g->dz * g->dz);
-	return prune_cells(m->stream, p, origin, cell, radius, counts, (flags & TG_FLAG_FAST) != 0, m->blocks);
+	b.radius = half * std::sqrt(g->dx * g->dx + g->dy * g->dy + g->dz * g->dz);
+	return b;
+}
+
+int prune_blocks_start(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t flags)
+{
+	const BlockGrid b = block_grid(m, g);
+	return prune_cells_start(m->stream, p, b.origin, b.cell, b.radius, b.counts, (flags & TG_FLAG_FAST) != 0, m->blocks);
+}
+
+int prune_blocks_finish(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t flags)
+{
+	const BlockGrid b = block_grid(m, g);
+	return prune_cells_finish(m->stream, p, b.origin, b.cell, b.counts, (flags & TG_FLAG_FAST) != 0, m->blocks);
+}
+
+int prune_blocks(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t flags)
+{
+	int rc = prune_blocks_start(m, p, g, flags);
+	if (rc) return rc;
+	return prune_blocks_finish(m, p, g, flags);
 }
 
 // Largest |p| over the grid's samples (the corners of its box).
@@ -359,6 +410,8 @@ int eval_blocks(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fl
 
 }  // namespace
 
+void destroy_mesher(tg_mesher* m);
+
 extern "C" int tg_mesher_create(tg_mesher** out)
 {
 	if (!out) return fail(TG_ERR_INVALID, "tg_mesher_create: null argument");
@@ -367,9 +420,11 @@ extern "C" int tg_mesher_create(tg_mesher** out)
 	auto* m = new tg_mesher;
 	hipError_t e = hipStreamCreate(&m->stream);
 	for (int i = 0; e == hipSuccess && i < 6; ++i) e = hipEventCreate(&m->ev[i]);
+	if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&m->tail), 16, hipHostMallocDefault);
 	if (e != hipSuccess)
 	{
-		delete m;
+		m->tail = nullptr;
+		destroy_mesher(m);
 		return hip_fail(e, "mesher stream/events");
 	}
 	*out = m;
@@ -386,9 +441,12 @@ int refuse_batch_owned(const tg_mesher* m, const char* what)
 	return TG_OK;
 }
 
+}  // namespace
+
 void destroy_mesher(tg_mesher* m)
 {
 	if (m->stream) (void)hipStreamSynchronize(m->stream);
+	if (m->tail) (void)hipHostFree(m->tail);
 	for (DevBuf* b : { &m->pool, &m->signs, &m->active, &m->word_count, &m->word_base, &m->quad_count, &m->quad_base,
 	                   &m->vertices, &m->triangles, &m->scan_tmp, &m->dense, &m->counters, &m->brick_flags,
 	                   &m->brick_active, &m->tile_mask, &m->block_list, &m->tile_list, &m->cubes, &m->normals,
@@ -400,8 +458,6 @@ void destroy_mesher(tg_mesher* m)
 	if (m->stream) (void)hipStreamDestroy(m->stream);
 	delete m;
 }
-
-}  // namespace
 
 extern "C" void tg_mesher_destroy(tg_mesher* m)
 {
@@ -1627,13 +1683,20 @@ extern "C" int tg_batch_export(tg_batch* b, const tg_program* const* programs, c
 	std::vector<BlockEvalArgs> args(count);
 	BatchArgs ba{};
 	uint64_t samples = 0;
+	// every tree's pruning is issued on its own stream before any is waited for
 	for (uint32_t i = 0; i < count; ++i)
 	{
 		tg_mesher* m = b->meshers[i];
 		m->have_grid = false;
 		m->nbricks = 0;
 		rc = layout(m, &grids[i]);
-		if (!rc) rc = prune_blocks(m, programs[i], &grids[i], flags);
+		if (!rc) rc = prune_blocks_start(m, programs[i], &grids[i], flags);
+		if (rc) return rc;
+	}
+	for (uint32_t i = 0; i < count; ++i)
+	{
+		tg_mesher* m = b->meshers[i];
+		rc = prune_blocks_finish(m, programs[i], &grids[i], flags);
 		if (rc) return rc;
 		if (!m->blocks.valid) return fail(TG_ERR_LIMIT, "tg_batch_export: a tree could not be pruned");
 		args[i] = block_args(m, programs[i], &grids[i], nullptr, nullptr, 0, b->work.as<unsigned long long>() + 2 * i);
@@ -1650,15 +1713,17 @@ extern "C" int tg_batch_export(tg_batch* b, const tg_program* const* programs, c
 		uint32_t tree, block;
 	};
 	std::vector<Item> order;
-	std::vector<BlockDesc> desc;
+	std::vector<std::vector<BlockDesc>> desc(count);
 	for (uint32_t i = 0; i < count; ++i)
 	{
-		const uint32_t nb = args[i].nblocks;
-		desc.resize(nb);
-		TG_CHECK_HIP(hipMemcpyAsync(desc.data(), args[i].desc, size_t(nb) * sizeof(BlockDesc), hipMemcpyDeviceToHost,
-			b->meshers[i]->stream), "block descriptors");
+		desc[i].resize(args[i].nblocks);
+		TG_CHECK_HIP(hipMemcpyAsync(desc[i].data(), args[i].desc, desc[i].size() * sizeof(BlockDesc),
+			hipMemcpyDeviceToHost, b->meshers[i]->stream), "block descriptors");
+	}
+	for (uint32_t i = 0; i < count; ++i)
+	{
 		TG_CHECK_HIP(hipStreamSynchronize(b->meshers[i]->stream), "block descriptors");
-		for (uint32_t k = 0; k < nb; ++k) order.push_back(Item{ double(desc[k].flops), i, k });
+		for (uint32_t k = 0; k < args[i].nblocks; ++k) order.push_back(Item{ double(desc[i][k].flops), i, k });
 	}
 	std::stable_sort(order.begin(), order.end(), [](const Item& x, const Item& y) { return x.cost > y.cost; });
 	std::vector<uint32_t> items(2 * order.size());
@@ -1692,11 +1757,27 @@ extern "C" int tg_batch_export(tg_batch* b, const tg_program* const* programs, c
 	double mesh_ms = 0;
 	uint64_t evals = 0;
 	double flops = 0;
+	// surface nets: each stage over every tree (on the trees' streams) before waiting
+	for (uint32_t i = 0; i < count; ++i)
+	{
+		rc = surface_nets_classify(b->meshers[i]);
+		if (rc) return rc;
+	}
+	for (uint32_t i = 0; i < count; ++i)
+	{
+		TG_CHECK_HIP(hipStreamSynchronize(b->meshers[i]->stream), "vertex count");
+		rc = surface_nets_vertices(b->meshers[i]);
+		if (rc) return rc;
+	}
+	for (uint32_t i = 0; i < count; ++i)
+	{
+		TG_CHECK_HIP(hipStreamSynchronize(b->meshers[i]->stream), "quad count");
+		rc = surface_nets_quads(b->meshers[i]);
+		if (rc) return rc;
+	}
 	for (uint32_t i = 0; i < count; ++i)
 	{
 		tg_mesher* m = b->meshers[i];
-		rc = surface_nets(m);
-		if (rc) return rc;
 		TG_CHECK_HIP(hipStreamSynchronize(m->stream), "batch mesh");
 		m->have_grid = true;
 		tg_export_stats& s = m->stats;

@@ -49,16 +49,109 @@ bool entry_levels(const DecodedProgram& d, std::vector<uint32_t>& order, std::ve
 
 }  // namespace
 
-int prune_cells(hipStream_t stream, const tg_program* p, const float origin[3], const float cell[3], float radius,
-	const uint32_t counts[3], bool fast, CellPrograms& out)
+namespace {
+
+// One emission pass (and, on the first, the level-parallel decision passes)
+// with the pool capacity out.state.capacity; the five counters are copied to
+// out.host (pinned) on `stream`.
+int prune_pass(hipStream_t stream, const tg_program* p, const float origin[3], const float cell[3],
+	const uint32_t counts[3], bool fast, CellPrograms& out, bool first)
+{
+	const DecodedProgram& d = p->decoded;
+	PruneState& ps = out.state;
+	int rc = out.pool.ensure(ps.capacity * sizeof(Entry));
+	if (rc) return rc;
+	hipError_t e = hipMemsetAsync(out.counters.ptr, 0, 64, stream);
+	if (e != hipSuccess) return hip_fail(e, "prune counters");
+	PruneArgs pa{};
+	pa.prog = static_cast<const Instr*>(p->device.ptr);
+	pa.entries = p->entries.as<const Entry>();
+	pa.n_entries = uint32_t(d.entries.size());
+	pa.n_binary = d.binary_entries;
+	pa.ox = origin[0]; pa.oy = origin[1]; pa.oz = origin[2];
+	pa.cx = cell[0]; pa.cy = cell[1]; pa.cz = cell[2];
+	pa.radius = ps.decide_radius;
+	pa.nx = counts[0]; pa.ny = counts[1]; pa.nz = counts[2];
+	pa.slots = std::max<uint32_t>(1, d.max_stack - 1);
+	pa.whole_flops = uint32_t(d.flops);
+	pa.pool = out.pool.as<Entry>();
+	pa.capacity = ps.capacity;
+	pa.desc = out.desc.as<BlockDesc>();
+	pa.counters = out.counters.as<unsigned long long>();
+	pa.scratch = out.scratch.as<uint64_t>();
+	pa.cells_per_wave = ps.cells_per_wave;
+	const uint64_t waves = ps.waves;
+	if (first && ps.level_parallel)
+	{
+		LevelArgs la{};
+		la.prog = pa.prog;
+		la.entries = pa.entries;
+		la.ngroups = uint32_t(waves);
+		la.cells_per_wave = ps.cells_per_wave;
+		la.ox = pa.ox; la.oy = pa.oy; la.oz = pa.oz;
+		la.cx = pa.cx; la.cy = pa.cy; la.cz = pa.cz;
+		la.radius = ps.decide_radius;
+		la.nx = pa.nx; la.ny = pa.ny; la.nz = pa.nz;
+		la.values = out.values.as<float>();
+		la.scratch = pa.scratch;
+		la.n_binary = pa.n_binary;
+		la.n_entries = pa.n_entries;
+		la.counts = out.lens.as<uint32_t>();
+		e = hipMemsetAsync(out.lens.ptr, 0, waves * 64 * 4, stream);
+		if (e != hipSuccess) return hip_fail(e, "length reset");
+		const size_t heights = ps.heights;
+		// values and decisions upward from height 1 (k_prune_level evaluates
+		// leaf operands in place), then liveness and lengths downward
+		for (size_t step = 1; step < 2 * heights; ++step)
+		{
+			const size_t h = step < heights ? step : 2 * heights - 1 - step;
+			la.liveness = step < heights ? 0 : 1;
+			la.list = out.levels.as<const uint32_t>() + ps.level_first[h];
+			la.count = ps.level_first[h + 1] - ps.level_first[h];
+			// a few cell groups per wave while enough waves remain to fill the GPU
+			la.groups_per_wave = 1;
+			while (la.groups_per_wave < 8 && uint64_t(la.count) * waves / (la.groups_per_wave * 2) >= 8192)
+				la.groups_per_wave *= 2;
+			e = fast ? fast::launch_prune_level(la, stream) : exact::launch_prune_level(la, stream);
+			if (e != hipSuccess) return hip_fail(e, "level pruning launch");
+		}
+	}
+	// masks, liveness and lengths stay in scratch for the rerun after a pool overflow
+	pa.masks_ready = ps.level_parallel ? 1 : 0;
+	pa.counts = out.lens.as<const uint32_t>();
+	// emission only: four waves per cell group while that leaves under ~8
+	// waves per SIMD
+	pa.splits = (ps.level_parallel && waves * 4 <= 8192) ? 4u : 1u;
+	e = fast ? fast::launch_prune_cells(pa, stream) : exact::launch_prune_cells(pa, stream);
+	if (e != hipSuccess) return hip_fail(e, "cell pruning launch");
+	e = hipMemcpyAsync(out.host, out.counters.ptr, 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream);
+	if (e != hipSuccess) return hip_fail(e, "cell pruning counters");
+	return TG_OK;
+}
+
+}  // namespace
+
+int prune_cells_start(hipStream_t stream, const tg_program* p, const float origin[3], const float cell[3],
+	float radius, const uint32_t counts[3], bool fast, CellPrograms& out)
 {
 	const DecodedProgram& d = p->decoded;
 	out.valid = false;
+	out.state.pending = false;
 	if (!d.lipschitz || d.entries.empty() || d.entries.size() >= 65535) return TG_OK;
-	const uint32_t whole_slots = std::max<uint32_t>(1, d.max_stack - 1);
 
 	const uint64_t ncells = uint64_t(counts[0]) * counts[1] * counts[2];
 	if (ncells == 0 || ncells >= (1ull << 31)) return TG_OK;
+	if (!out.host)
+	{
+		hipError_t e = hipHostMalloc(&out.host, 64, hipHostMallocDefault);
+		if (e != hipSuccess)
+		{
+			out.host = nullptr;
+			return hip_fail(e, "pinned pruning counters");
+		}
+	}
+	PruneState& ps = out.state;
+	ps.ncells = ncells;
 	// decision radius: the cell radius with the Lipschitz tolerance and the
 	// rounding slack for points of the cells' box
 	float pmax = 0.0f;
@@ -67,116 +160,77 @@ int prune_cells(hipStream_t stream, const tg_program* p, const float origin[3], 
 		const float a = std::fabs(origin[i]), b = std::fabs(origin[i] + float(counts[i]) * cell[i]);
 		pmax += std::max(a, b) * std::max(a, b);
 	}
-	const float decide_radius = radius * (1.0f + 1.0e-5f) + rounding_slack(d, std::sqrt(pmax));
+	ps.decide_radius = radius * (1.0f + 1.0e-5f) + rounding_slack(d, std::sqrt(pmax));
 	// Fewer cells per wave only when there are very few cells (every wave
 	// gathers the whole program, so smaller groups cost L2 traffic).
 	uint32_t cpw = 64;
 	while (cpw > 16 && (ncells + cpw - 1) / cpw < 256) cpw >>= 1;
-	const uint64_t waves = (ncells + cpw - 1) / cpw;
+	ps.cells_per_wave = cpw;
+	ps.waves = (ncells + cpw - 1) / cpw;
 	int rc = out.desc.ensure(ncells * sizeof(BlockDesc));
 	if (!rc) rc = out.counters.ensure(64);
-	if (!rc) rc = out.scratch.ensure(waves * prune_scratch_words(d.binary_entries, uint32_t(d.entries.size())) * 8);
+	if (!rc) rc = out.scratch.ensure(ps.waves * prune_scratch_words(d.binary_entries, uint32_t(d.entries.size())) * 8);
 	if (rc) return rc;
 	// Level-parallel decision pass when the heights are few and the entry
 	// values fit: it replaces the per-wave walk over the whole program.
-	std::vector<uint32_t> level_order, level_first;
-	bool level_parallel = entry_levels(d, level_order, level_first) &&
-	                      uint64_t(d.entries.size()) * waves * 64 * 4 <= kMaxValueBytes;
-	if (level_parallel)
+	std::vector<uint32_t>& level_order = ps.level_order;
+	ps.level_parallel = entry_levels(d, level_order, ps.level_first) &&
+	                    uint64_t(d.entries.size()) * ps.waves * 64 * 4 <= kMaxValueBytes;
+	if (ps.level_parallel)
 	{
-		rc = out.values.ensure(uint64_t(d.entries.size()) * waves * 64 * 4);
+		ps.heights = ps.level_first.size() - 1;
+		rc = out.values.ensure(uint64_t(d.entries.size()) * ps.waves * 64 * 4);
 		if (!rc) rc = out.levels.ensure(level_order.size() * 4);
-		if (!rc) rc = out.lens.ensure(waves * 64 * 4);
+		if (!rc) rc = out.lens.ensure(ps.waves * 64 * 4);
 		if (rc) return rc;
 		hipError_t e = hipMemcpyAsync(out.levels.ptr, level_order.data(), level_order.size() * 4,
 			hipMemcpyHostToDevice, stream);
 		if (e != hipSuccess) return hip_fail(e, "level upload");
 	}
 	// First guess: a quarter of the whole program per cell, at most 1 GiB; grown
-	// to the exact total and rerun if any cell did not fit.
-	uint64_t capacity = std::max<uint64_t>(out.pool.bytes / sizeof(Entry),
+	// to the exact total and rerun if any cell did not fit (prune_cells_finish).
+	ps.capacity = std::max<uint64_t>(out.pool.bytes / sizeof(Entry),
 		std::min<uint64_t>(ncells * std::max<size_t>(d.entries.size() / 4, 16), (1ull << 30) / sizeof(Entry)));
-	for (int attempt = 0; attempt < 2; ++attempt)
+	rc = prune_pass(stream, p, origin, cell, counts, fast, out, true);
+	if (rc) return rc;
+	ps.pending = true;
+	return TG_OK;
+}
+
+int prune_cells_finish(hipStream_t stream, const tg_program* p, const float origin[3], const float cell[3],
+	const uint32_t counts[3], bool fast, CellPrograms& out)
+{
+	PruneState& ps = out.state;
+	if (!ps.pending) return TG_OK;
+	ps.pending = false;
+	hipError_t e = hipStreamSynchronize(stream);
+	if (e != hipSuccess) return hip_fail(e, "cell pruning");
+	const unsigned long long* c = out.host;
+	if (c[0] > ps.capacity && c[0] * sizeof(Entry) <= (size_t(8) << 30))
 	{
-		rc = out.pool.ensure(capacity * sizeof(Entry));
+		// some cells did not fit: rerun the emission with the exact total
+		ps.capacity = c[0];
+		int rc = prune_pass(stream, p, origin, cell, counts, fast, out, false);
 		if (rc) return rc;
-		hipError_t e = hipMemsetAsync(out.counters.ptr, 0, 64, stream);
-		if (e != hipSuccess) return hip_fail(e, "prune counters");
-		PruneArgs pa{};
-		pa.prog = static_cast<const Instr*>(p->device.ptr);
-		pa.entries = p->entries.as<const Entry>();
-		pa.n_entries = uint32_t(d.entries.size());
-		pa.n_binary = d.binary_entries;
-		pa.ox = origin[0]; pa.oy = origin[1]; pa.oz = origin[2];
-		pa.cx = cell[0]; pa.cy = cell[1]; pa.cz = cell[2];
-		pa.radius = decide_radius;
-		pa.nx = counts[0]; pa.ny = counts[1]; pa.nz = counts[2];
-		pa.slots = whole_slots;
-		pa.whole_flops = uint32_t(d.flops);
-		pa.pool = out.pool.as<Entry>();
-		pa.capacity = capacity;
-		pa.desc = out.desc.as<BlockDesc>();
-		pa.counters = out.counters.as<unsigned long long>();
-		pa.scratch = out.scratch.as<uint64_t>();
-		pa.cells_per_wave = cpw;
-		if (attempt == 0 && level_parallel)
-		{
-			LevelArgs la{};
-			la.prog = pa.prog;
-			la.entries = pa.entries;
-			la.ngroups = uint32_t(waves);
-			la.cells_per_wave = cpw;
-			la.ox = pa.ox; la.oy = pa.oy; la.oz = pa.oz;
-			la.cx = pa.cx; la.cy = pa.cy; la.cz = pa.cz;
-			la.radius = decide_radius;
-			la.nx = pa.nx; la.ny = pa.ny; la.nz = pa.nz;
-			la.values = out.values.as<float>();
-			la.scratch = pa.scratch;
-			la.n_binary = pa.n_binary;
-			la.n_entries = pa.n_entries;
-			la.counts = out.lens.as<uint32_t>();
-			e = hipMemsetAsync(out.lens.ptr, 0, waves * 64 * 4, stream);
-			if (e != hipSuccess) return hip_fail(e, "length reset");
-			const size_t heights = level_first.size() - 1;
-			// values and decisions upward from height 1 (k_prune_level evaluates
-			// leaf operands in place), then liveness and lengths downward
-			for (size_t step = 1; step < 2 * heights; ++step)
-			{
-				const size_t h = step < heights ? step : 2 * heights - 1 - step;
-				la.liveness = step < heights ? 0 : 1;
-				la.list = out.levels.as<const uint32_t>() + level_first[h];
-				la.count = level_first[h + 1] - level_first[h];
-				// a few cell groups per wave while enough waves remain to fill the GPU
-				la.groups_per_wave = 1;
-				while (la.groups_per_wave < 8 && uint64_t(la.count) * waves / (la.groups_per_wave * 2) >= 8192)
-					la.groups_per_wave *= 2;
-				e = fast ? fast::launch_prune_level(la, stream) : exact::launch_prune_level(la, stream);
-				if (e != hipSuccess) return hip_fail(e, "level pruning launch");
-			}
-		}
-		// masks, liveness and lengths stay in scratch for the rerun after a pool overflow
-		pa.masks_ready = level_parallel ? 1 : 0;
-		pa.counts = out.lens.as<const uint32_t>();
-		// emission only: four waves per cell group while that leaves under ~8
-		// waves per SIMD
-		pa.splits = (level_parallel && waves * 4 <= 8192) ? 4u : 1u;
-		e = fast ? fast::launch_prune_cells(pa, stream) : exact::launch_prune_cells(pa, stream);
-		if (e != hipSuccess) return hip_fail(e, "cell pruning launch");
-		unsigned long long c[5] = { 0, 0, 0, 0, 0 };
-		e = hipMemcpyAsync(c, out.counters.ptr, sizeof(c), hipMemcpyDeviceToHost, stream);
-		if (e == hipSuccess) e = hipStreamSynchronize(stream);
+		e = hipStreamSynchronize(stream);
 		if (e != hipSuccess) return hip_fail(e, "cell pruning");
-		out.max_stack = uint32_t(c[1]);
-		out.max_binary = uint32_t(c[2]);
-		out.max_len = uint32_t(c[3]);
-		out.whole_cells = c[4];
-		out.entries = c[0];
-		if (c[0] <= capacity || c[0] * sizeof(Entry) > (size_t(8) << 30)) break;
-		capacity = c[0];
 	}
-	out.ncells = ncells;
+	out.max_stack = uint32_t(c[1]);
+	out.max_binary = uint32_t(c[2]);
+	out.max_len = uint32_t(c[3]);
+	out.whole_cells = c[4];
+	out.entries = c[0];
+	out.ncells = ps.ncells;
 	out.valid = true;
 	return TG_OK;
+}
+
+int prune_cells(hipStream_t stream, const tg_program* p, const float origin[3], const float cell[3], float radius,
+	const uint32_t counts[3], bool fast, CellPrograms& out)
+{
+	int rc = prune_cells_start(stream, p, origin, cell, radius, counts, fast, out);
+	if (rc) return rc;
+	return prune_cells_finish(stream, p, origin, cell, counts, fast, out);
 }
 
 }  // namespace tg
