@@ -115,8 +115,8 @@ typedef struct tg_export_stats
 	double total_ms;           /* whole export on the device stream */
 	uint64_t samples;          /* (sx+1)(sy+1)(sz+1) */
 	uint64_t tiles_evaluated;  /* 8^3-sample tiles interpreted */
-	uint64_t exact_evals;      /* samples actually interpreted */
-	double flops;              /* algorithmic FP32 work executed (cost table) */
+	uint64_t exact_evals;      /* lanes interpreted (whole 8^3 tiles, edge tiles' lanes past the grid included) */
+	double flops;              /* algorithmic FP32 work (cost table) of the programs run at the grid's samples */
 	uint64_t bricks_total;     /* culled export: 8^3-cube bricks of the grid (0 otherwise) */
 	uint64_t bricks_flagged;   /* culled export: bricks the surrogate kept for exact evaluation */
 	double prune_ms;           /* block pruning, and for a culled export brick selection (HIP events; inside eval_ms) */

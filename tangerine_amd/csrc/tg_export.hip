@@ -632,7 +632,8 @@ int run_export(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fla
 			ga.ntiles = sel.listed;
 		}
 		evals = uint64_t(ga.ntiles) * kTileSamples;
-		flops = double(evals) * p->decoded.flops;
+		// work over grid samples (all of them, or the listed tiles' lanes)
+		flops = double(culled ? evals : uint64_t(ma.nx) * ma.ny * ma.nz) * p->decoded.flops;
 		hipError_t e = (flags & TG_FLAG_FAST) ? fast::launch_tiles(ga, 8, m->stream)
 		                                      : exact::launch_tiles(ga, 8, m->stream);
 		if (e != hipSuccess) return hip_fail(e, "tile evaluation launch");

@@ -146,7 +146,7 @@ struct BlockEvalArgs
 	uint32_t max_depth;          // deepest block program's stack (host: sizes `slots`)
 	int prune_tiles;             // 0: evaluate block programs unpruned per tile
 	uint32_t min_prune_len;      // blocks with shorter programs skip the per-tile pruning
-	unsigned long long* work;    // [0] samples interpreted, [1] flops units executed
+	unsigned long long* work;    // [0] lanes interpreted (whole tiles), [1] cost units over the grid's samples
 	const Entry* whole;          // whole program, for blocks whose pruning overflowed
 	uint32_t n_whole;
 	uint32_t n_whole_binary;
@@ -196,14 +196,15 @@ struct CertifyArgs
 
 // ---- sphere tracing -------------------------------------------------------------
 
-constexpr int kTraceLevels = 4;
+constexpr int kTraceLevels = 5;
 
 struct TraceLevel
 {
 	const BlockDesc* desc;   // null: level absent
 	const Entry* pool;       // this level's pruned programs
-	uint32_t n;              // cells per axis (cubic cells over the domain)
+	uint32_t n;              // cells per axis (cubic cells over the level's cube)
 	float cell;              // cell edge
+	float o[3];              // the cube's lowest corner
 };
 
 struct TraceArgs
@@ -214,7 +215,7 @@ struct TraceArgs
 	uint32_t whole_flops;
 	TraceLevel levels[kTraceLevels];  // coarse to fine
 	int nlevels;
-	float dom[3];                     // domain origin
+
 	// rays: camera (width * height pixels) or arrays (nrays)
 	int camera;
 	float eye[3], fwd[3], right[3], up[3];

@@ -122,6 +122,21 @@ bool decode_words(const uint32_t* w, size_t n, DecodedProgram& out, std::string&
 	std::vector<uint32_t> starts;  // subtree start record per stack entry
 	double last_brush_norm = 1.0;  // transform norm of the brush just emitted
 	double last_brush_extent = 0.0;  // its extent in its own frame (scaled by a following ScaleField)
+	double last_centre[3] = { 0, 0, 0 };  // world position of its frame's origin
+	double last_radius = 0.0;        // its parameter magnitudes in its own frame
+	double widen = 0.0;              // largest blend threshold / flate radius
+	bool any_plane = false, any_brush = false;
+	double box_lo[3] = { 1e300, 1e300, 1e300 }, box_hi[3] = { -1e300, -1e300, -1e300 };
+	auto close_brush = [&](double scale) {
+		const double r = last_radius * scale;
+		for (int k = 0; k < 3; ++k)
+		{
+			box_lo[k] = std::min(box_lo[k], last_centre[k] - r);
+			box_hi[k] = std::max(box_hi[k], last_centre[k] + r);
+		}
+		any_brush = true;
+	};
+	bool brush_open = false;  // a brush whose field scale may still follow
 	size_t pc = 0;
 	bool stopped = false;
 	while (pc < n)
@@ -131,6 +146,8 @@ bool decode_words(const uint32_t* w, size_t n, DecodedProgram& out, std::string&
 		{
 			if (last_brush_norm > 1.0 + 1e-5) out.lipschitz = false;
 			last_brush_norm = 1.0;
+			if (brush_open) close_brush(1.0);
+			brush_open = false;
 		}
 		if (op == OP_STOP)
 		{
@@ -184,6 +201,39 @@ bool decode_words(const uint32_t* w, size_t n, DecodedProgram& out, std::string&
 				if (op == OP_CONE) mag += std::fabs(double(as_float(w[pc - np])) * as_float(w[pc - np + 1]));
 				last_brush_extent = mag;
 				out.leaf_extent = std::max(out.leaf_extent, float(mag));
+				// frame origin in world space: q = A p + b vanishes at p = -A^-1 b
+				double radius = 0.0;
+				for (int i = 0; i < np; ++i) radius += std::fabs(double(as_float(w[pc - np + i])));
+				if (op == OP_CONE) radius += std::fabs(double(as_float(w[pc - np])) * as_float(w[pc - np + 1]));
+				last_radius = radius;
+				if (op == OP_PLANE) any_plane = true;
+				const double b[3] = { pending[3], pending[7], pending[11] };
+				if (pending_kind == XF_MATRIX)
+				{
+					const double A[3][3] = { { pending[0], pending[1], pending[2] }, { pending[4], pending[5], pending[6] },
+					                         { pending[8], pending[9], pending[10] } };
+					const double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) -
+					                   A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
+					                   A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+					if (std::fabs(det) > 1e-30)
+					{
+						// Cramer's rule for A c = -b
+						for (int k = 0; k < 3; ++k)
+						{
+							double M[3][3];
+							for (int r = 0; r < 3; ++r)
+								for (int c = 0; c < 3; ++c) M[r][c] = (c == k) ? -b[r] : A[r][c];
+							last_centre[k] = (M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) -
+							                  M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+							                  M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0])) / det;
+						}
+					}
+					else
+						any_plane = true;  // degenerate frame: no finite placement
+				}
+				else
+					for (int k = 0; k < 3; ++k) last_centre[k] = -b[k];
+				brush_open = true;
 			}
 			out.max_stack = std::max<uint32_t>(out.max_stack, uint32_t(starts.size()));
 			pending_kind = XF_NONE;
@@ -211,6 +261,7 @@ bool decode_words(const uint32_t* w, size_t n, DecodedProgram& out, std::string&
 					return false;
 				}
 				const float t = as_float(w[pc++]);
+				widen = std::max(widen, std::fabs(double(t)));
 				s.w[3] = as_word(t);
 				s.w[4] = as_word(0.25f / t);
 				// exact build: RN(0.25 / t) and 4 t in double, low word first
@@ -249,8 +300,12 @@ bool decode_words(const uint32_t* w, size_t n, DecodedProgram& out, std::string&
 				if (ls > 1.0 + 1e-5) out.lipschitz = false;
 				out.leaf_extent = std::max(out.leaf_extent,
 				                           float(last_brush_extent * std::fabs(double(as_float(u.w[1])))));
+				if (brush_open) close_brush(std::fabs(double(as_float(u.w[1]))));
+				brush_open = false;
 				last_brush_norm = 1.0;
 			}
+			else
+				widen = std::max(widen, 2.0 * std::fabs(double(as_float(u.w[1]))));
 			out.instrs.push_back(u);
 			continue;
 		}
@@ -292,6 +347,13 @@ bool decode_words(const uint32_t* w, size_t n, DecodedProgram& out, std::string&
 		err = "program terminated without a Stop word";
 		return false;
 	}
+	out.scene_finite = any_brush && !any_plane;
+	if (out.scene_finite)
+		for (int k = 0; k < 3; ++k)
+		{
+			out.scene_lo[k] = float(box_lo[k] - widen);
+			out.scene_hi[k] = float(box_hi[k] + widen);
+		}
 	if (out.max_stack > uint32_t(kMaxStack))
 	{
 		err = "program needs " + std::to_string(out.max_stack) + " stack slots; the device interpreter holds " +

@@ -29,6 +29,13 @@ struct DecodedProgram
 	// translation| + the sum of its parameter magnitudes); bounds the operand
 	// magnitudes a brush's arithmetic sees besides the point itself.
 	float leaf_extent = 0.0f;
+	// A box holding every brush's surface (each brush as a ball around its
+	// frame's origin, radius = field scale x the sum of its parameter
+	// magnitudes), widened by the largest blend threshold and flate radius;
+	// scene_finite is false with a plane.  Only a placement hint (the tracer's
+	// cell domain), never a bound the evaluation relies on.
+	float scene_lo[3] = { 0, 0, 0 }, scene_hi[3] = { 0, 0, 0 };
+	bool scene_finite = false;
 };
 
 // Validates stack discipline and opcodes the way SDFInterpreter::Eval would

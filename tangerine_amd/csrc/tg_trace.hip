@@ -47,22 +47,32 @@ struct tg_tracer
 
 namespace {
 
-// Cell hierarchy over the cube [lo, lo + size]^3: levels of 8, 16, 32, 64 cells.
-int build_levels(hipStream_t stream, const tg_program* p, const float lo[3], float size, bool fast,
+// Cell hierarchy: an outer level of 8^3 cells over `outer` (the cube holding
+// the eye), then 8, 16, 32, 64 cells per axis over `inner` (the scene's cube,
+// where rays spend their steps).  Each level is a cube (lowest corner, edge).
+struct CubeDomain
+{
+	float lo[3];
+	float size;
+};
+
+int build_levels(hipStream_t stream, const tg_program* p, const CubeDomain& outer, const CubeDomain& inner, bool fast,
 	CellPrograms* levels, TraceArgs& ta)
 {
 	ta.nlevels = 0;
 	for (int L = 0; L < kTraceLevels; ++L)
 	{
-		const uint32_t n = 8u << L;
-		const float cell = size / float(n);
+		const CubeDomain& dom = L == 0 ? outer : inner;
+		const uint32_t n = L == 0 ? 8u : (8u << (L - 1));
+		const float cell = dom.size / float(n);
 		const float c3[3] = { cell, cell, cell };
 		const uint32_t counts[3] = { n, n, n };
 		const float radius = 0.5f * cell * 1.7320508f;
-		int rc = prune_cells(stream, p, lo, c3, radius, counts, fast, levels[L]);
+		int rc = prune_cells(stream, p, dom.lo, c3, radius, counts, fast, levels[L]);
 		if (rc) return rc;
 		if (!levels[L].valid) return TG_OK;
-		ta.levels[L] = TraceLevel{ levels[L].desc.as<const BlockDesc>(), levels[L].pool.as<const Entry>(), n, cell };
+		ta.levels[L] = TraceLevel{ levels[L].desc.as<const BlockDesc>(), levels[L].pool.as<const Entry>(), n, cell,
+		                           { dom.lo[0], dom.lo[1], dom.lo[2] } };
 		ta.nlevels = L + 1;
 	}
 	return TG_OK;
@@ -183,23 +193,34 @@ extern "C" int tg_tracer_render(tg_tracer* tr, const tg_program* p, const tg_cam
 	TG_CHECK_HIP(hipEventRecord(tr->ev[0], tr->stream), "event");
 	if (!(flags & TG_FLAG_NO_PRUNE))
 	{
-		float lo[3], size;
+		// Outer level: the origin-centred cube that holds the eye.  Inner
+		// levels: the caller's domain, or the cube around the program's scene
+		// box (outer otherwise).
+		CubeDomain outer{}, inner{};
+		float reach = 1.0f;
+		for (int i = 0; i < 3; ++i) reach = std::max(reach, std::fabs(cam->eye[i]) + 1.0f);
+		outer.size = 2.0f * reach;
+		for (int i = 0; i < 3; ++i) outer.lo[i] = -reach;
+		inner = outer;
+		const DecodedProgram& d = p->decoded;
 		if (tr->dom_size > 0.0f)
 		{
-			for (int i = 0; i < 3; ++i) lo[i] = tr->dom_lo[i];
-			size = tr->dom_size;
+			for (int i = 0; i < 3; ++i) inner.lo[i] = tr->dom_lo[i];
+			inner.size = tr->dom_size;
 		}
-		else
+		else if (d.scene_finite)
 		{
-			// Default domain: the origin-centred cube that holds the eye.
-			float reach = 1.0f;
-			for (int i = 0; i < 3; ++i) reach = std::max(reach, std::fabs(cam->eye[i]) + 1.0f);
-			size = 2.0f * reach;
-			for (int i = 0; i < 3; ++i) lo[i] = -reach;
+			float edge = 0.0f;
+			for (int i = 0; i < 3; ++i) edge = std::max(edge, d.scene_hi[i] - d.scene_lo[i]);
+			edge = edge * 1.01f + 1e-3f;
+			if (edge < outer.size)
+			{
+				inner.size = edge;
+				for (int i = 0; i < 3; ++i) inner.lo[i] = 0.5f * (d.scene_lo[i] + d.scene_hi[i]) - 0.5f * edge;
+			}
 		}
-		rc = build_levels(tr->stream, p, lo, size, fast, tr->levels, ta);
+		rc = build_levels(tr->stream, p, outer, inner, fast, tr->levels, ta);
 		if (rc) return rc;
-		for (int i = 0; i < 3; ++i) ta.dom[i] = lo[i];
 	}
 	TG_CHECK_HIP(hipEventRecord(tr->ev[1], tr->stream), "event");
 	ta.camera = 1;
