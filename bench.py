@@ -207,15 +207,16 @@ def export_leg(mesher, prog, origin, step, cells, args, fast: bool, tune: bool, 
     lib = _lib.load()
     tuning = None
     if args.tuning:
-        vals = [int(v) for v in args.tuning.split(",")] + [0]
-        mesher.set_tuning(vals[0], vals[1], vals[2])
-        tuning = {"points_per_lane": vals[0], "waves": vals[1], "min_prune_len": vals[2], "source": "--tuning"}
+        vals = [int(v) for v in args.tuning.split(",")] + [0, 0]
+        mesher.set_tuning(vals[0], vals[1], vals[3], vals[2])
+        tuning = {"points_per_lane": vals[0], "waves": vals[1], "workgroups_per_cu": vals[2],
+                  "min_prune_len": vals[3], "source": "--tuning"}
     elif tune and not args.no_prune:
         t0 = time.perf_counter()
         tuning = mesher.autotune(prog, origin, step, cells, fast=fast)
         tuning["wall_ms"] = 1000.0 * (time.perf_counter() - t0)
         log(f"{label} autotune: {tuning['points_per_lane']} points/lane, {tuning['waves']} waves, "
-            f"{tuning['eval_ms']:.2f} ms")
+            f"{tuning['workgroups_per_cu']} workgroups/CU, {tuning['eval_ms']:.2f} ms")
     for i in range(args.warmup):
         mesher.export(prog, origin, step, cells, fast=fast, prune=not args.no_prune)
     _lib.check(lib.tg_device_synchronize(), "sync")
@@ -405,9 +406,10 @@ def main(argv=None) -> int:
     ap.add_argument("--no-prune", action="store_true", help="interpret the whole program in every tile")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
-    ap.add_argument("--tuning", default="", metavar="P,W[,L]",
-                    help="block-kernel launch (points per lane, waves per workgroup, shortest block program pruned "
-                         "per tile) instead of autotuning")
+    ap.add_argument("--tuning", default="", metavar="P,W[,G[,L]]",
+                    help="block-evaluation launch (points per lane, waves per workgroup, persistent workgroups per "
+                         "CU or 0 for a workgroup per block, shortest block program pruned per tile) instead of "
+                         "autotuning")
     ap.add_argument("--sur-samples", type=int, default=2 * 1024 * 1024)
     ap.add_argument("--sur-epochs", type=int, default=16)
     ap.add_argument("--sur-clamp", type=float, default=0.5)
@@ -496,7 +498,8 @@ def main(argv=None) -> int:
                        "autotune": ftune, "roofline": roofline(fst["flops"], fms, kernel + " (fast numerics)")}
         # back to the exact export (and its launch shape) for the surrogate leg's judge
         if tuning:
-            mesher.set_tuning(tuning["points_per_lane"], tuning["waves"], tuning.get("min_prune_len", 0))
+            mesher.set_tuning(tuning["points_per_lane"], tuning["waves"], tuning.get("min_prune_len", 0),
+                              tuning.get("workgroups_per_cu", 0))
         mesher.export(prog, origin, step, cells, prune=prune)
     if "surrogate" in legs:
         out["surrogate_culled"] = surrogate_leg(mesher, prog, wl, origin, step, cells, args)

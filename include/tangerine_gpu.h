@@ -355,28 +355,35 @@ int tg_mesher_export_sodapop(tg_mesher* mesher, const tg_program* program, const
 int tg_mesher_copy_normals(const tg_mesher* mesher, float* normals);
 
 /* ---- block-evaluator tuning (no reference counterpart) ----------------------
- * The pruned block kernel has two launch parameters: points per lane (2, 4
- * or 8 samples interpreted together by one lane: more work per fetched
- * instruction, more registers and LDS) and waves per workgroup (1, 2, 4),
- * and one work parameter: the shortest block program still pruned per tile.
+ * The pruned block evaluation has three launch parameters: points per lane
+ * (2, 4 or 8 samples interpreted together by one lane: more work per fetched
+ * instruction, more registers and LDS), waves per workgroup (1, 2, 4) and
+ * workgroups per CU: 0 launches a workgroup per 32^3-sample block with its
+ * decision tables and tile programs in LDS; 1..16 launches that many
+ * persistent workgroups per CU, which take blocks from a queue and keep the
+ * tables and tile programs in global scratch (LDS: the spill region only).
+ * One work parameter: the shortest block program still pruned per tile.
  * tg_mesher_autotune prunes `program` for `grid`, times every launch on the
  * device, then the threshold at the fastest launch, and keeps the best for
- * later exports on this mesher. */
+ * later exports on this mesher.  Every setting writes the same samples. */
 typedef struct tg_tune_result
 {
 	uint32_t points_per_lane;     /* chosen */
 	uint32_t waves;
+	uint32_t workgroups_per_cu;   /* chosen: 0 = a workgroup per block */
 	uint32_t min_prune_len;       /* chosen: shorter block programs skip per-tile pruning */
 	double eval_ms;               /* block kernel time of the chosen configuration */
 	uint32_t candidates;
-	uint32_t candidate_points[9]; /* configuration that actually ran (LDS may force a smaller one) */
-	uint32_t candidate_waves[9];
-	double candidate_ms[9];
+	uint32_t candidate_points[24]; /* configuration that actually ran (LDS may force fewer points per lane) */
+	uint32_t candidate_waves[24];
+	uint32_t candidate_workgroups_per_cu[24];
+	double candidate_ms[24];
 	double candidate_len_ms[3];   /* min_prune_len 0, 16, 48 at the chosen launch */
 } tg_tune_result;
 int tg_mesher_autotune(tg_mesher* mesher, const tg_program* program, const tg_grid* grid, uint32_t flags,
                        tg_tune_result* out);
-int tg_mesher_set_tuning(tg_mesher* mesher, uint32_t points_per_lane, uint32_t waves, uint32_t min_prune_len);
+int tg_mesher_set_tuning(tg_mesher* mesher, uint32_t points_per_lane, uint32_t waves, uint32_t workgroups_per_cu,
+                         uint32_t min_prune_len);
 
 /* ---- batched export (no reference counterpart; closest: one Sodapop
  * MeshingJob per model on Scheduler workers, sodapop.cpp:214-224,

@@ -54,9 +54,10 @@ class TgSurrogateStats(C.Structure):
 
 
 class TgTuneResult(C.Structure):
-    _fields_ = [("points_per_lane", _u32), ("waves", _u32), ("min_prune_len", _u32), ("eval_ms", C.c_double),
-                ("candidates", _u32), ("candidate_points", _u32 * 9), ("candidate_waves", _u32 * 9),
-                ("candidate_ms", C.c_double * 9), ("candidate_len_ms", C.c_double * 3)]
+    _fields_ = [("points_per_lane", _u32), ("waves", _u32), ("workgroups_per_cu", _u32), ("min_prune_len", _u32),
+                ("eval_ms", C.c_double), ("candidates", _u32), ("candidate_points", _u32 * 24),
+                ("candidate_waves", _u32 * 24), ("candidate_workgroups_per_cu", _u32 * 24),
+                ("candidate_ms", C.c_double * 24), ("candidate_len_ms", C.c_double * 3)]
 
 
 class TgBatchStats(C.Structure):
@@ -144,7 +145,7 @@ SIGNATURES = {
     "tg_mesher_copy_grid": (C.c_int, [_p, _fp]),
     "tg_mesher_stats": (C.c_int, [_p, C.POINTER(TgExportStats)]),
     "tg_mesher_autotune": (C.c_int, [_p, _p, C.POINTER(TgGrid), _u32, C.POINTER(TgTuneResult)]),
-    "tg_mesher_set_tuning": (C.c_int, [_p, _u32, _u32, _u32]),
+    "tg_mesher_set_tuning": (C.c_int, [_p, _u32, _u32, _u32, _u32]),
     "tg_mesher_write_stl": (C.c_int, [_p, _p, C.c_char_p, _f, _u32]),
     "tg_mesher_write_ply": (C.c_int, [_p, _p, C.c_char_p, _f, _u32]),
     "tg_mesher_grid_get": (C.c_int, [_p, C.POINTER(TgGrid)]),

@@ -51,7 +51,12 @@ struct tg_mesher
 	bool have_grid = false;
 	tg_export_stats stats{};
 	int points_per_lane = 4;      // block kernel: points per lane (2, 4, 8; tg_mesher_autotune)
-	int waves = 4;                // block kernel: waves per workgroup (1, 2, 4)
+	int waves = 2;                // block kernel: waves per workgroup (1, 2, 4)
+	int workgroups_per_cu = 8;    // 0: a workgroup per block (LDS tables); else persistent, global tables
+	int cus = 0;                  // compute units of the device
+	DevBuf persist_args, persist_head, persist_tables;  // the persistent launch's argument copy, queue head, tables
+	DevBuf order_keys, order_values, order, order_tmp;   // its queue: blocks by program cost
+	int last_workgroups = 0;
 	uint32_t min_prune_len = 0;   // block kernel: shorter block programs skip per-tile pruning
 	int last_points_per_lane = 0;
 	int last_waves = 0;
@@ -374,6 +379,78 @@ int eval_blocks(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fl
 	TG_CHECK_HIP(hipMemsetAsync(m->counters.ptr, 0, 64, m->stream), "counter reset");
 	BlockEvalArgs ba = block_args(m, p, g, tile_mask, block_list, n_listed, work);
 	int pts = m->points_per_lane;
+	if (m->workgroups_per_cu > 0)
+	{
+		// Persistent launch: workgroups take blocks from a queue; each keeps its
+		// block's decision tables and tile programs in its own slice of global
+		// scratch, LDS holds only tile descriptors and the spill region.
+		const size_t per_wg = kMaxLds / size_t(m->workgroups_per_cu);
+		const int waves = m->waves;
+		while (pts > 2 && block_lds_bytes_global(block_spill_slots(ba.max_depth, pts), pts, waves) > per_wg) pts >>= 1;
+		if (block_lds_bytes_global(block_spill_slots(ba.max_depth, pts), pts, waves) > per_wg)
+			return fail(TG_ERR_LIMIT, "block evaluation: spill region too large for the workgroups per CU");
+		ba.slots = block_spill_slots(ba.max_depth, pts, (per_wg - kTileInfoBytes) / size_t(waves));
+		ba.waves = waves;
+		BatchArgs q{};
+		q.ntrees = 1;
+		q.items = nullptr;  // item k: block k of the one tree
+		q.total = ba.nblocks;
+		q.slots = ba.slots;
+		q.max_binary = ba.max_binary;
+		q.max_len = ba.max_len;
+		q.max_depth = ba.max_depth;
+		q.table_words = block_global_words(ba.max_binary, ba.max_len);
+		const uint32_t wgs = uint32_t(std::max(1, m->cus * m->workgroups_per_cu));
+		rc = m->persist_args.ensure(sizeof(BlockEvalArgs));
+		if (!rc) rc = m->persist_head.ensure(256);
+		if (!rc) rc = m->persist_tables.ensure(size_t(wgs) * q.table_words * 4);
+		if (rc) return rc;
+		q.trees = m->persist_args.as<const BlockEvalArgs>();
+		q.head = m->persist_head.as<unsigned int>();
+		q.tables = m->persist_tables.as<uint32_t>();
+		// the queue takes the most expensive block programs first, so that its
+		// tail holds short ones
+		{
+			const uint32_t nb = ba.nblocks;
+			size_t tmp = 0;
+			TG_CHECK_HIP(blocks_by_cost(ba.desc, ba.block_list, nb, nullptr, nullptr, nullptr, nullptr, tmp, m->stream),
+				"sort sizing");
+			rc = m->order_keys.ensure(std::max<size_t>(nb, 1) * 8);
+			if (!rc) rc = m->order_values.ensure(std::max<size_t>(nb, 1) * 4);
+			if (!rc) rc = m->order.ensure(std::max<size_t>(nb, 1) * 4);
+			if (!rc) rc = m->order_tmp.ensure(std::max<size_t>(tmp, 1));
+			if (rc) return rc;
+			tmp = m->order_tmp.bytes;
+			TG_CHECK_HIP(blocks_by_cost(ba.desc, ba.block_list, nb, m->order_keys.as<uint32_t>(),
+				m->order_values.as<uint32_t>(), m->order.as<uint32_t>(), m->order_tmp.ptr, tmp, m->stream), "block order");
+			ba.block_list = m->order.as<const uint32_t>();
+		}
+		TG_CHECK_HIP(hipMemcpyAsync(m->persist_args.ptr, &ba, sizeof(ba), hipMemcpyHostToDevice, m->stream), "upload");
+		m->last_points_per_lane = pts;
+		m->last_waves = waves;
+		m->last_workgroups = int(wgs);
+		m->last_max_len = ba.max_len;
+		m->last_max_binary = ba.max_binary;
+		m->last_lds = uint32_t(block_lds_bytes_global(ba.slots, pts, waves));
+		const bool fast = (flags & TG_FLAG_FAST) != 0;
+		hipError_t e = fast ? fast::launch_batch_reset(q.head, nullptr, 0, m->stream)
+		                    : exact::launch_batch_reset(q.head, nullptr, 0, m->stream);
+		if (e != hipSuccess) return hip_fail(e, "queue reset");
+		TG_CHECK_HIP(hipEventRecord(m->ev[4], m->stream), "event");
+		e = fast ? fast::launch_batch(q, pts, uint32_t(waves), wgs, m->stream)
+		         : exact::launch_batch(q, pts, uint32_t(waves), wgs, m->stream);
+		if (e != hipSuccess) return hip_fail(e, "block evaluation launch");
+		TG_CHECK_HIP(hipEventRecord(m->ev[5], m->stream), "event");
+		unsigned long long wk[2] = { 0, 0 };
+		TG_CHECK_HIP(hipMemcpyAsync(wk, work, 16, hipMemcpyDeviceToHost, m->stream), "work read");
+		TG_CHECK_HIP(hipStreamSynchronize(m->stream), "block evaluation");
+		float kms = 0;
+		TG_CHECK_HIP(hipEventElapsedTime(&kms, m->ev[4], m->ev[5]), "timing");
+		m->last_block_ms = kms;
+		*evals = wk[0];
+		*flops = double(wk[1]);
+		return TG_OK;
+	}
 	// Fit the LDS: fewer points per lane first, then fewer waves.
 	auto lds_for = [&](int p, int w) {
 		return block_lds_bytes(block_spill_slots(ba.max_depth, p), p, w, ba.max_binary, ba.max_len);
@@ -389,6 +466,7 @@ int eval_blocks(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fl
 	}
 	m->last_points_per_lane = pts;
 	m->last_waves = ba.waves;
+	m->last_workgroups = 0;
 	m->last_max_len = ba.max_len;
 	m->last_max_binary = ba.max_binary;
 	m->last_lds = uint32_t(block_lds_bytes(ba.slots, pts, ba.waves, ba.max_binary, ba.max_len));
@@ -418,7 +496,10 @@ extern "C" int tg_mesher_create(tg_mesher** out)
 	int rc = ensure_device();
 	if (rc) return rc;
 	auto* m = new tg_mesher;
-	hipError_t e = hipStreamCreate(&m->stream);
+	int dev = 0;
+	hipError_t e = hipGetDevice(&dev);
+	if (e == hipSuccess) e = hipDeviceGetAttribute(&m->cus, hipDeviceAttributeMultiprocessorCount, dev);
+	if (e == hipSuccess) e = hipStreamCreate(&m->stream);
 	for (int i = 0; e == hipSuccess && i < 6; ++i) e = hipEventCreate(&m->ev[i]);
 	if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&m->tail), 16, hipHostMallocDefault);
 	if (e != hipSuccess)
@@ -450,7 +531,8 @@ void destroy_mesher(tg_mesher* m)
 	for (DevBuf* b : { &m->pool, &m->signs, &m->active, &m->word_count, &m->word_base, &m->quad_count, &m->quad_base,
 	                   &m->vertices, &m->triangles, &m->scan_tmp, &m->dense, &m->counters, &m->brick_flags,
 	                   &m->brick_active, &m->tile_mask, &m->block_list, &m->tile_list, &m->cubes, &m->normals,
-	                   &m->exchange })
+	                   &m->exchange, &m->persist_args, &m->persist_head, &m->persist_tables, &m->order_keys,
+	                   &m->order_values, &m->order, &m->order_tmp })
 		b->release();
 	m->blocks.release();
 	for (hipEvent_t e : m->ev)
@@ -726,14 +808,17 @@ extern "C" int tg_mesher_active_bricks(tg_mesher* m, uint8_t* out, uint64_t capa
 	return TG_OK;
 }
 
-extern "C" int tg_mesher_set_tuning(tg_mesher* m, uint32_t points_per_lane, uint32_t waves, uint32_t min_prune_len)
+extern "C" int tg_mesher_set_tuning(tg_mesher* m, uint32_t points_per_lane, uint32_t waves,
+	uint32_t workgroups_per_cu, uint32_t min_prune_len)
 {
 	if (!m || !(points_per_lane == 2 || points_per_lane == 4 || points_per_lane == 8) ||
-		!(waves == 1 || waves == 2 || waves == 4))
-		return fail(TG_ERR_INVALID, "tg_mesher_set_tuning: points per lane in {2,4,8}, waves in {1,2,4}");
+		!(waves == 1 || waves == 2 || waves == 4) || workgroups_per_cu > 16)
+		return fail(TG_ERR_INVALID,
+			"tg_mesher_set_tuning: points per lane in {2,4,8}, waves in {1,2,4}, workgroups per CU 0..16");
 	if (int rc = refuse_batch_owned(m, "tg_mesher_set_tuning")) return rc;
 	m->points_per_lane = int(points_per_lane);
 	m->waves = int(waves);
+	m->workgroups_per_cu = int(workgroups_per_cu);
 	m->min_prune_len = min_prune_len;
 	return TG_OK;
 }
@@ -754,14 +839,17 @@ extern "C" int tg_mesher_autotune(tg_mesher* m, const tg_program* p, const tg_gr
 	if (rc) return rc;
 	if (!m->blocks.valid) return fail(TG_ERR_INVALID, "tg_mesher_autotune: program not prunable here");
 	tg_tune_result res{};
+	// (waves, workgroups per CU): a workgroup per block (0) with LDS tables,
+	// or persistent workgroups with global tables
 	const int pts_c[3] = { 2, 4, 8 };
-	const int waves_c[3] = { 1, 2, 4 };
+	const int shape_c[8][2] = { { 1, 0 }, { 2, 0 }, { 4, 0 }, { 2, 6 }, { 2, 8 }, { 2, 12 }, { 4, 4 }, { 4, 6 } };
 	double best = 1e300;
 	for (int pi = 0; pi < 3; ++pi)
-		for (int wi = 0; wi < 3; ++wi)
+		for (int wi = 0; wi < 8; ++wi)
 		{
 			m->points_per_lane = pts_c[pi];
-			m->waves = waves_c[wi];
+			m->waves = shape_c[wi][0];
+			m->workgroups_per_cu = shape_c[wi][1];
 			double ms = 0.0;
 			for (int rep = 0; rep < 3; ++rep)
 			{
@@ -777,20 +865,23 @@ extern "C" int tg_mesher_autotune(tg_mesher* m, const tg_program* p, const tg_gr
 				if (rep > 0) ms += t;  // first run warms caches
 			}
 			ms *= 0.5;
-			const int k = pi * 3 + wi;
+			const int k = pi * 8 + wi;
 			res.candidate_ms[k] = ms;
 			// a candidate the LDS could not hold ran as a smaller one: record what ran
 			res.candidate_points[k] = uint32_t(m->last_points_per_lane);
 			res.candidate_waves[k] = uint32_t(m->last_waves);
+			res.candidate_workgroups_per_cu[k] = uint32_t(m->workgroups_per_cu);
 			if (ms < best)
 			{
 				best = ms;
 				res.points_per_lane = uint32_t(m->last_points_per_lane);
 				res.waves = uint32_t(m->last_waves);
+				res.workgroups_per_cu = uint32_t(m->workgroups_per_cu);
 			}
 		}
 	m->points_per_lane = int(res.points_per_lane);
 	m->waves = int(res.waves);
+	m->workgroups_per_cu = int(res.workgroups_per_cu);
 	// then the shortest block program worth pruning per tile, at that launch
 	const uint32_t len_c[3] = { 0, 16, 48 };
 	res.min_prune_len = 0;
@@ -815,7 +906,7 @@ extern "C" int tg_mesher_autotune(tg_mesher* m, const tg_program* p, const tg_gr
 		}
 	}
 	m->min_prune_len = res.min_prune_len;
-	res.candidates = 9;
+	res.candidates = 24;
 	res.eval_ms = best;
 	if (out) *out = res;
 	return TG_OK;

@@ -161,7 +161,8 @@ struct BlockEvalArgs
 struct BatchArgs
 {
 	const BlockEvalArgs* trees;  // per tree (device array)
-	const uint32_t* items;       // [2 * total]: tree, block of each queue item, in queue order
+	const uint32_t* items;       // [2 * total]: tree, block of each queue item, in queue order (null: item k is
+	                             // block k of tree 0)
 	uint32_t ntrees;
 	uint32_t total;              // queue items
 	unsigned int* head;          // queue head (reset to 0 before each launch)

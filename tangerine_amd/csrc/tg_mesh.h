@@ -6,6 +6,8 @@
 
 namespace tg {
 
+struct BlockDesc;
+
 struct MeshArgs
 {
 	// grid (same meaning as isosurface::regular_grid_t)
@@ -41,5 +43,10 @@ hipError_t mesh_quad_write(const MeshArgs& a, uint64_t nverts, hipStream_t s);
 hipError_t mesh_pool_to_dense(const MeshArgs& a, float* dense, uint32_t k0, uint32_t nk, hipStream_t s);
 hipError_t mesh_dense_to_pool(const MeshArgs& a, const float* dense, hipStream_t s);
 hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, void* temp, size_t& temp_bytes, hipStream_t s);
+// Blocks (all n, or the n of `list`) ordered by their pruned program's cost,
+// most expensive first, into `order`; keys needs 2n words, values n.  With
+// temp = null only sizes temp_bytes.
+hipError_t blocks_by_cost(const BlockDesc* desc, const uint32_t* list, uint32_t n, uint32_t* keys,
+	uint32_t* values, uint32_t* order, void* temp, size_t& temp_bytes, hipStream_t s);
 
 }  // namespace tg

@@ -31,6 +31,7 @@
 #include <hipcub/hipcub.hpp>
 #include <stdint.h>
 
+#include "tg_kernels.h"
 #include "tg_mesh.h"
 #include "tg_types.h"
 
@@ -382,6 +383,37 @@ hipError_t mesh_dense_to_pool(const MeshArgs& a, const float* dense, hipStream_t
 hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, void* temp, size_t& temp_bytes, hipStream_t s)
 {
 	return hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, in, out, int(n), s);
+}
+
+namespace {
+
+// Thread per block (or per listed block): its program's cost (keys) and its
+// index (values) for the sort below.
+__global__ void __launch_bounds__(256) k_block_costs(const BlockDesc* desc, const uint32_t* list, uint32_t n,
+	uint32_t* keys, uint32_t* values)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const uint32_t b = list ? list[i] : i;
+	keys[i] = desc[b].flops;
+	values[i] = b;
+}
+
+}  // namespace
+
+hipError_t blocks_by_cost(const BlockDesc* desc, const uint32_t* list, uint32_t n, uint32_t* keys,
+	uint32_t* values, uint32_t* order, void* temp, size_t& temp_bytes, hipStream_t s)
+{
+	uint32_t* keys_out = keys + n;
+	if (temp)
+	{
+		if (n == 0) return hipSuccess;
+		hipLaunchKernelGGL(k_block_costs, dim3((n + 255) / 256), dim3(256), 0, s, desc, list, n, keys, values);
+		const hipError_t e = hipGetLastError();
+		if (e != hipSuccess) return e;
+	}
+	return hipcub::DeviceRadixSort::SortPairsDescending(temp, temp_bytes, keys, keys_out, values, order, int(n), 0,
+		32, s);
 }
 
 }  // namespace tg

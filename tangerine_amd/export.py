@@ -63,16 +63,19 @@ class Mesher:
         check(self.lib.tg_mesher_autotune(self.h, program.handle, C.byref(g), _lib.TG_FLAG_FAST if fast else 0,
                                           C.byref(r)), "tg_mesher_autotune")
         return {
-            "points_per_lane": r.points_per_lane, "waves": r.waves, "min_prune_len": r.min_prune_len,
-            "eval_ms": r.eval_ms,
+            "points_per_lane": r.points_per_lane, "waves": r.waves, "workgroups_per_cu": r.workgroups_per_cu,
+            "min_prune_len": r.min_prune_len, "eval_ms": r.eval_ms,
             "candidates": [{"points_per_lane": r.candidate_points[i], "waves": r.candidate_waves[i],
-                            "ms": r.candidate_ms[i]} for i in range(r.candidates)],
+                            "workgroups_per_cu": r.candidate_workgroups_per_cu[i], "ms": r.candidate_ms[i]}
+                           for i in range(r.candidates)],
             "min_prune_len_ms": {str(v): r.candidate_len_ms[i] for i, v in enumerate((0, 16, 48))},
         }
 
-    def set_tuning(self, points_per_lane: int, waves: int, min_prune_len: int = 0) -> None:
-        check(self.lib.tg_mesher_set_tuning(self.h, int(points_per_lane), int(waves), int(min_prune_len)),
-              "tg_mesher_set_tuning")
+    def set_tuning(self, points_per_lane: int, waves: int, min_prune_len: int = 0, workgroups_per_cu: int = 0) -> None:
+        """Block-evaluation launch: workgroups_per_cu 0 = a workgroup per
+        block (LDS tables), 1..16 = persistent workgroups (global tables)."""
+        check(self.lib.tg_mesher_set_tuning(self.h, int(points_per_lane), int(waves), int(workgroups_per_cu),
+                                            int(min_prune_len)), "tg_mesher_set_tuning")
 
     def _bricks(self, fn, name) -> np.ndarray:
         n = C.c_uint64()

@@ -105,7 +105,7 @@ def test_batch_owned_meshers_refuse_relayout(gpu):
     g = _lib.grid_struct(*grids[1])
     assert lib.tg_mesher_export(h, progs[1].handle, C.byref(g), 0) == -1
     assert b"belongs to a tg_batch" in lib.tg_last_error()
-    assert lib.tg_mesher_set_tuning(h, 2, 4, 0) == -1
+    assert lib.tg_mesher_set_tuning(h, 2, 4, 0, 0) == -1
     lib.tg_mesher_destroy(h)  # refused: the handle stays valid
     assert b"belongs to a tg_batch" in lib.tg_last_error()
     assert b.replay() > 0

@@ -127,24 +127,28 @@ def test_long_chains(mesher, oracle, length):
 
 
 def test_every_tuning_gives_the_same_grid(mesher):
-    """Every (points per lane, waves, shortest pruned block program) setting
-    of the block kernel writes the same bits, and the autotuner's choice is
-    one of the timed candidates."""
+    """Every (points per lane, waves, workgroups per CU, shortest pruned block
+    program) setting of the block evaluation writes the same bits (a
+    workgroup per block with LDS tables, or persistent workgroups with global
+    tables), and the autotuner's choice is one of the timed candidates."""
     from tangerine_amd import scenes
     prog = scenes.build_sdf(scenes.random_csg_spec(1, 256)).program()
     origin, step, cells = scenes.grid_for(-6.0, 6.0, 100)
     ref = None
-    launches = [(p, w, 0) for p in (2, 4, 8) for w in (1, 2, 4)] + [(4, 2, 16), (2, 4, 1000)]
-    for pts, waves, min_len in launches:
-        mesher.set_tuning(pts, waves, min_len)
+    launches = ([(p, w, 0, 0) for p in (2, 4, 8) for w in (1, 2, 4)] +
+                [(p, w, 0, g) for p in (2, 4, 8) for w, g in ((2, 8), (4, 4), (1, 16))] +
+                [(4, 2, 16, 0), (2, 4, 1000, 0), (4, 2, 16, 8)])
+    for pts, waves, min_len, wgs in launches:
+        mesher.set_tuning(pts, waves, min_len, wgs)
         mesher.export(prog, origin, step, cells)
         g = bits(mesher.grid())
         if ref is None:
             ref = g
-        assert np.array_equal(g, ref), (pts, waves, min_len)
+        assert np.array_equal(g, ref), (pts, waves, min_len, wgs)
     r = mesher.autotune(prog, origin, step, cells)
-    assert len(r["candidates"]) == 9 and r["eval_ms"] > 0
-    assert (r["points_per_lane"], r["waves"]) in {(c["points_per_lane"], c["waves"]) for c in r["candidates"]}
+    assert len(r["candidates"]) == 24 and r["eval_ms"] > 0
+    assert (r["points_per_lane"], r["waves"], r["workgroups_per_cu"]) in {
+        (c["points_per_lane"], c["waves"], c["workgroups_per_cu"]) for c in r["candidates"]}
     mesher.export(prog, origin, step, cells)
     assert np.array_equal(bits(mesher.grid()), ref)
-    mesher.set_tuning(4, 4, 0)
+    mesher.set_tuning(4, 2, 0, 8)
