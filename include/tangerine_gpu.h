@@ -98,7 +98,9 @@ typedef struct tg_grid
 typedef struct tg_program_info
 {
 	uint64_t instructions;     /* device records, including Stop */
-	uint32_t max_stack;        /* interpreter stack high-water mark */
+	uint32_t max_stack;        /* device interpreter stack high-water mark: the reference's StackSize, or for
+	                            * programs deeper than 16 slots the depth after the decoder runs each set
+	                            * operator's deeper operand first (values unchanged) */
 	uint32_t brushes;
 	uint32_t set_ops;
 	uint32_t lipschitz;        /* 1 if every primitive is a distance bound (no ellipsoid) */
@@ -128,6 +130,7 @@ typedef struct tg_export_stats
 	uint32_t block_max_len;    /* longest pruned block program (entries) and its binary entries */
 	uint32_t block_max_binary;
 	uint32_t block_lds_bytes;  /* LDS per workgroup of the block kernel */
+	uint32_t block_workgroups; /* persistent workgroups of that launch (0: a workgroup per block) */
 } tg_export_stats;
 
 typedef struct tg_surrogate_stats
@@ -194,6 +197,9 @@ int tg_node_compile(const tg_node* node, uint32_t* words, uint64_t capacity, uin
 
 /* ---- device programs ------------------------------------------------------ */
 int tg_program_from_node(const tg_node* node, tg_program** out);
+/* Any stack depth (the reference sizes its stack from StackSize); TG_ERR_LIMIT
+ * only for programs that need more than 16 slots in every operand order (a
+ * balanced tree of 65,536 brushes or more). */
 int tg_program_from_words(const uint32_t* words, uint64_t count, tg_program** out);
 int tg_program_info_get(const tg_program* program, tg_program_info* out);
 /* Host only: validates words as tg_program_from_words does and reports the

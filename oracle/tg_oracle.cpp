@@ -626,8 +626,13 @@ static inline float wf(const uint32_t* w, size_t i)
 
 static float interpret(const uint32_t* w, size_t n, V3 ep, bool* ok)
 {
-	float stack[256];
-	int sp = 0;
+	// the reference sizes its stack from the tree (StackSize, no cap): 256
+	// slots here, grown on the heap for deeper programs
+	float small[256];
+	std::vector<float> big;
+	float* stack = small;
+	size_t cap = 256;
+	size_t sp = 0;
 	size_t pc = 0;
 	V3 p = ep;
 	*ok = true;
@@ -645,7 +650,13 @@ static float interpret(const uint32_t* w, size_t n, V3 ep, bool* ok)
 			float prm[3] = { 0, 0, 0 };
 			for (int i = 0; i < np; ++i) prm[i] = wf(w, pc + i);
 			pc += np;
-			if (sp >= 256) { *ok = false; return 0.0f; }
+			if (sp >= cap)
+			{
+				big.resize(2 * cap);
+				if (stack == small) std::memcpy(big.data(), small, sizeof(small));
+				stack = big.data();
+				cap = big.size();
+			}
 			stack[sp++] = eval_brush(op, prm, p);
 			p = ep;
 			break;

@@ -44,7 +44,7 @@ class TgExportStats(C.Structure):
                 ("bricks_total", _u64), ("bricks_flagged", _u64), ("prune_ms", C.c_double),
                 ("cull_threshold", C.c_double), ("bricks_certified", _u64), ("block_ms", C.c_double),
                 ("points_per_lane", _u32), ("waves", _u32), ("block_max_len", _u32), ("block_max_binary", _u32),
-                ("block_lds_bytes", _u32)]
+                ("block_lds_bytes", _u32), ("block_workgroups", _u32)]
 
 
 class TgSurrogateStats(C.Structure):

@@ -658,8 +658,9 @@ int run_export(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fla
 	const bool certify = culled && (!sur || (flags & TG_FLAG_CERTIFY));
 	if (culled)
 	{
-		if (!p->decoded.lipschitz)
-			return fail(TG_ERR_PROGRAM, "culled export needs a 1-Lipschitz program (no ellipsoid, no stretching scale)");
+		if (!p->decoded.prunable())
+			return fail(TG_ERR_PROGRAM, "culled export needs a prunable program (1-Lipschitz: no ellipsoid, no "
+			                            "stretching scale; at most 128 nested blend / flate / field-scale operators)");
 		if (sur && !(surrogate_threshold(sur, 0.0f) >= 0.0f))
 			return fail(TG_ERR_INVALID, "culled export: surrogate is not calibrated");
 	}
@@ -684,7 +685,7 @@ int run_export(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fla
 
 	TG_CHECK_HIP(hipEventRecord(m->ev[0], m->stream), "event");
 	m->blocks.valid = false;
-	if (!(flags & TG_FLAG_NO_PRUNE) && p->decoded.lipschitz)
+	if (!(flags & TG_FLAG_NO_PRUNE) && p->decoded.prunable())
 	{
 		rc = prune_blocks(m, p, g, flags);
 		if (rc) return rc;
@@ -746,6 +747,7 @@ int run_export(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fla
 	s.block_max_len = blocks ? m->last_max_len : 0u;
 	s.block_max_binary = blocks ? m->last_max_binary : 0u;
 	s.block_lds_bytes = blocks ? m->last_lds : 0u;
+	s.block_workgroups = blocks ? uint32_t(m->last_workgroups) : 0u;
 	if (culled)
 	{
 		s.bricks_total = m->nbricks;
@@ -830,7 +832,7 @@ extern "C" int tg_mesher_autotune(tg_mesher* m, const tg_program* p, const tg_gr
 	if (int rc = refuse_batch_owned(m, "tg_mesher_autotune")) return rc;
 	int rc = validate_grid(g);
 	if (rc) return rc;
-	if ((flags & TG_FLAG_NO_PRUNE) || !p->decoded.lipschitz)
+	if ((flags & TG_FLAG_NO_PRUNE) || !p->decoded.prunable())
 		return fail(TG_ERR_INVALID, "tg_mesher_autotune: tunes the pruned block evaluator only");
 	rc = layout(m, g);
 	if (rc) return rc;
@@ -1744,8 +1746,9 @@ extern "C" int tg_batch_export(tg_batch* b, const tg_program* const* programs, c
 	for (uint32_t i = 0; i < count; ++i)
 	{
 		if (!programs[i]) return fail(TG_ERR_INVALID, "tg_batch_export: null program");
-		if (!programs[i]->decoded.lipschitz)
-			return fail(TG_ERR_PROGRAM, "tg_batch_export: every tree must be 1-Lipschitz (pruned evaluation)");
+		if (!programs[i]->decoded.prunable())
+			return fail(TG_ERR_PROGRAM, "tg_batch_export: every tree must be prunable (1-Lipschitz, at most 128 "
+			                            "nested blend / flate / field-scale operators)");
 		int rc = validate_grid(&grids[i]);
 		if (rc) return rc;
 	}

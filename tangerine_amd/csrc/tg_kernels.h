@@ -206,6 +206,7 @@ struct TraceLevel
 	uint32_t n;              // cells per axis (cubic cells over the level's cube)
 	float cell;              // cell edge
 	float o[3];              // the cube's lowest corner
+	float inv;               // RN(1 / cell)
 };
 
 struct TraceArgs

@@ -1,5 +1,6 @@
 #include "tg_program.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -59,7 +60,81 @@ double spectral_norm3(const float* rows)
 	return std::sqrt(lambda);
 }
 
+// Reorders the records (Stop last) so that every set operator evaluates the
+// operand that needs the deeper stack first (Sethi-Ullman order), flagging the
+// operators whose right operand now runs first (kSwapOperands).  The stack
+// then holds at most the tree's Strahler number of values: a chain of any
+// length needs 2, and 17 needs a balanced tree of 65,536 brushes.  Values do
+// not change: a brush record carries its own point transform, so a subtree
+// evaluates to the same bits wherever it runs, and a flagged operator still
+// combines (left, right).  Returns the stack high-water mark.
+uint32_t reorder_operands(std::vector<Instr>& in)
+{
+	const size_t n = in.size() - 1;
+	std::vector<uint32_t> need(n), first(n), second(n);
+	std::vector<uint8_t> swap(n, 0);
+	for (size_t i = 0; i < n; ++i)
+	{
+		const uint32_t op = instr_op(in[i]);
+		if (is_brush(op))
+			need[i] = 1;
+		else if (is_binary(op))
+		{
+			// roots: the left operand ends right before the right one starts
+			const uint32_t a = in[i].w[2] - 1, b = uint32_t(i) - 1;
+			swap[i] = need[b] > need[a];
+			first[i] = swap[i] ? b : a;
+			second[i] = swap[i] ? a : b;
+			need[i] = std::max(std::max(need[a], need[b]), std::min(need[a], need[b]) + 1);
+		}
+		else
+		{
+			first[i] = uint32_t(i) - 1;  // flate / field scale
+			need[i] = need[i - 1];
+		}
+	}
+	// post-order walk from the root (the last record before Stop)
+	std::vector<Instr> out;
+	out.reserve(in.size());
+	std::vector<uint32_t> start(n);  // new index of each subtree's first record
+	struct Frame
+	{
+		uint32_t node, state;
+	};
+	std::vector<Frame> todo;
+	todo.push_back(Frame{ uint32_t(n - 1), 0 });
+	while (!todo.empty())
+	{
+		const uint32_t i = todo.back().node, state = todo.back().state;
+		const uint32_t op = instr_op(in[i]);
+		if (state == 0) start[i] = uint32_t(out.size());
+		const uint32_t arity = is_brush(op) ? 0u : is_binary(op) ? 2u : 1u;
+		if (state < arity)
+		{
+			todo.back().state = state + 1;
+			todo.push_back(Frame{ state == 0 ? first[i] : second[i], 0 });
+			continue;
+		}
+		Instr r = in[i];
+		if (arity == 2)
+		{
+			r.w[0] = (r.w[0] & ~kSwapOperands) | (swap[i] ? kSwapOperands : 0u);
+			r.w[1] = start[first[i]];
+			r.w[2] = start[second[i]];
+		}
+		out.push_back(r);
+		todo.pop_back();
+	}
+	out.push_back(in[n]);
+	in.swap(out);
+	return need[n - 1];
+}
+
 }  // namespace
+
+constexpr uint32_t kMaxRoundingDepth = 128;
+
+bool DecodedProgram::prunable() const { return lipschitz && rounding_depth <= kMaxRoundingDepth; }
 
 double instr_flops(const Instr& in)
 {
@@ -99,9 +174,10 @@ double instr_flops(const Instr& in)
 // a few dozen operations of the primitive, the divides and square roots
 // included) errs by at most K u (|p| + leaf extent) in world units, u =
 // 2^-24, with K = 64 taken generously; set operators propagate the larger
-// operand error (each is 1-Lipschitz in either operand) plus u |value|, and
-// flate / field scale add u |value|, which the value-proportional 1e-5 term
-// covers for any tree depth below 160.  Hence slack = 2 K u (pmax + extent),
+// operand error (each is 1-Lipschitz in either operand; min, max and negation
+// round nothing, a blend adds u |value|), and flate / field scale add
+// u |value|, which the value-proportional 1e-5 term covers for up to 160 such
+// roundings on a path (programs with more than 128 are not pruned).  Hence slack = 2 K u (pmax + extent),
 // with pmax bounding |p| over the domain; it never goes below 2e-4.  The
 // oracle's float64 interpreter measures the per-brush constant on random and
 // adversarial trees (tests/test_oracle.py).
@@ -354,34 +430,42 @@ bool decode_words(const uint32_t* w, size_t n, DecodedProgram& out, std::string&
 			out.scene_lo[k] = float(box_lo[k] - widen);
 			out.scene_hi[k] = float(box_hi[k] + widen);
 		}
+	if (out.max_stack > uint32_t(kMaxStack)) out.max_stack = reorder_operands(out.instrs);
 	if (out.max_stack > uint32_t(kMaxStack))
 	{
-		err = "program needs " + std::to_string(out.max_stack) + " stack slots; the device interpreter holds " +
-		      std::to_string(kMaxStack);
+		err = "program needs " + std::to_string(out.max_stack) + " stack slots in any operand order; the device "
+		      "interpreter holds " + std::to_string(kMaxStack);
 		return false;
 	}
 	// Record cost in bits 16..23 of word 0 (device-side work accounting), and
 	// the entry view with operand roots for per-cell pruning.
 	uint32_t ord = 0;
+	std::vector<uint32_t> rdepth(out.instrs.size(), 0);
 	for (size_t i = 0; i < out.instrs.size(); ++i)
 	{
 		Instr& in = out.instrs[i];
 		const double c = instr_flops(in);
 		out.flops += c;
-		in.w[0] = (in.w[0] & 0xffffu) | (uint32_t(c) << 16);
+		in.w[0] = (in.w[0] & (0xffffu | kSwapOperands)) | (uint32_t(c) << 16);
 		const uint32_t op = instr_op(in);
 		if (op == OP_STOP) break;
-		HostEntry e{ uint32_t(i) | (instr_kind(in) << 22) | (op << 24), 0, 0, 0 };
+		HostEntry e{ uint32_t(i) | (instr_kind(in) << 22) | (op << 24) | (in.w[0] & kSwapOperands), 0, 0, 0 };
 		if (is_binary(op))
 		{
-			e.l = in.w[2] - 1;   // the left operand ends right before the right one starts
-			e.r = uint32_t(i) - 1;
+			// the operand evaluated first ends right before the second one starts
+			const uint32_t below = in.w[2] - 1, top = uint32_t(i) - 1;
+			const bool swapped = (in.w[0] & kSwapOperands) != 0;
+			e.l = swapped ? top : below;
+			e.r = swapped ? below : top;
 			e.ord = ord++;
+			rdepth[i] = std::max(rdepth[e.l], rdepth[e.r]) + (is_blend(op) ? 1u : 0u);
 		}
 		else if (op == OP_FLATE || op == OP_SCALE_FIELD)
 		{
 			e.l = uint32_t(i) - 1;
+			rdepth[i] = rdepth[e.l] + 1;
 		}
+		out.rounding_depth = std::max(out.rounding_depth, rdepth[i]);
 		out.entries.push_back(e);
 	}
 	out.binary_entries = ord;

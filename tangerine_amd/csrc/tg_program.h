@@ -24,6 +24,12 @@ struct DecodedProgram
 	uint32_t brushes = 0;
 	uint32_t set_ops = 0;
 	bool lipschitz = true;       // false if any ellipsoid (not a distance bound)
+	// Most blend / flate / field-scale operators on one root-to-brush path:
+	// each rounds its value once, and the pruning decisions' value-proportional
+	// margin (1e-5 (|a| + |b|), about 167 roundings of 2^-24) covers at most
+	// kMaxRoundingDepth of them.
+	uint32_t rounding_depth = 0;
+	bool prunable() const;       // lipschitz, rounding_depth <= kMaxRoundingDepth
 	double flops = 0.0;          // algorithmic FP32 work per evaluation (cost table)
 	// Largest world-space size of a brush: its field scale times (|its frame
 	// translation| + the sum of its parameter magnitudes); bounds the operand

@@ -137,7 +137,7 @@ int prune_cells_start(hipStream_t stream, const tg_program* p, const float origi
 	const DecodedProgram& d = p->decoded;
 	out.valid = false;
 	out.state.pending = false;
-	if (!d.lipschitz || d.entries.empty() || d.entries.size() >= 65535) return TG_OK;
+	if (!d.prunable() || d.entries.empty() || d.entries.size() >= 65535) return TG_OK;
 
 	const uint64_t ncells = uint64_t(counts[0]) * counts[1] * counts[2];
 	if (ncells == 0 || ncells >= (1ull << 31)) return TG_OK;

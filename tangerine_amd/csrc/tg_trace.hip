@@ -72,7 +72,7 @@ int build_levels(hipStream_t stream, const tg_program* p, const CubeDomain& oute
 		if (rc) return rc;
 		if (!levels[L].valid) return TG_OK;
 		ta.levels[L] = TraceLevel{ levels[L].desc.as<const BlockDesc>(), levels[L].pool.as<const Entry>(), n, cell,
-		                           { dom.lo[0], dom.lo[1], dom.lo[2] } };
+		                           { dom.lo[0], dom.lo[1], dom.lo[2] }, 1.0f / cell };
 		ta.nlevels = L + 1;
 	}
 	return TG_OK;

@@ -46,7 +46,8 @@ enum XformKind : uint32_t
 
 // One instruction = 16 dwords.
 //   brush:  w[0] = opcode | kind << 8;  w[1..12] = 3x4 matrix rows; w[13..15] = params
-//   set op: w[0] = opcode;  w[1] = lhs subtree start; w[2] = rhs subtree start;
+//   set op: w[0] = opcode | kSwapOperands (if set);  w[1] = start of the operand
+//           evaluated first, w[2] = start of the operand evaluated second;
 //           f[3] = threshold; f[4] = 0.25 / threshold (blend only);
 //           w[5..6] = RN(0.25 / threshold), w[7..8] = 4 threshold as doubles,
 //           low word first (blend only)
@@ -58,6 +59,13 @@ struct Instr
 };
 
 static_assert(sizeof(Instr) == 64, "instruction record must be 64 bytes");
+
+// Set-operator flag (bit 30 of a record's word 0 and of an entry word): the
+// right operand was evaluated first, so the left operand is on top of the
+// stack.  Programs deeper than the device stack are reordered so that every
+// operator evaluates its deeper operand first (Sethi-Ullman order); the
+// operator still combines (left, right) as the source program does.
+constexpr uint32_t kSwapOperands = 1u << 30;
 
 inline uint32_t instr_op(const Instr& I) { return I.w[0] & 0xffu; }
 inline uint32_t instr_kind(const Instr& I) { return (I.w[0] >> 8) & 3u; }
@@ -74,6 +82,6 @@ struct Grid
 
 constexpr int kTile = 8;          // samples per tile edge (8^3 = 512 samples / wave)
 constexpr int kTileSamples = 512;
-constexpr int kMaxStack = 16;     // interpreter stack slots per point
+constexpr int kMaxStack = 16;     // interpreter stack slots per point (after operand reordering)
 
 }  // namespace tg

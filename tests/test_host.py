@@ -89,17 +89,34 @@ def test_rejects_malformed_programs():
         assert _lib.last_error()
 
 
-def test_rejects_deep_stacks():
-    """A right-leaning chain of differences needs one stack slot per level;
-    beyond 16 the device interpreter refuses with a size error."""
+def test_deep_stacks_are_reordered():
+    """A right-leaning chain of differences needs one reference stack slot
+    per level (StackSize 21 for 20 levels, 301 for 300); the decoder runs each
+    operator's deeper operand first, so the device needs 2 slots at any
+    length.  A balanced tree needs log2(brushes) + 1 slots in any order:
+    32,768 brushes fit the 16 device slots as they are, 65,536 (17) are
+    refused with a size error."""
     from tangerine_amd import _lib, sdf
-    node = sdf.sphere(1)
-    for i in range(20):
-        node = sdf.sphere(1).diff(node)  # diff never swaps: right-deep
-    assert node.stack_size() == 21
-    w = node.compile()
-    h = C.c_void_p()
-    rc = _lib.load().tg_program_from_words(w.ctypes.data_as(_lib._u32p), w.size, C.byref(h))
+    from oracle.interp64 import SPHERE, STOP, UNION
+    for levels, ref in ((20, 21), (300, 301)):
+        node = sdf.sphere(1)
+        for i in range(levels):
+            node = sdf.sphere(1).diff(node)  # right-deep
+        assert node.stack_size() == ref
+        info = sdf.inspect_words(node.compile())
+        assert info["max_stack"] == 2 and info["brushes"] == levels + 1
+
+    def balanced(k):
+        one = np.array([SPHERE, np.float32(0.5).view(np.uint32)], dtype=np.uint32)
+        w = one
+        for _ in range(k):
+            w = np.concatenate([w, w, np.array([UNION], dtype=np.uint32)])
+        return np.concatenate([w, np.array([STOP], dtype=np.uint32)])
+
+    assert sdf.inspect_words(balanced(15))["max_stack"] == 16
+    w = balanced(16)
+    info = _lib.TgProgramInfo()
+    rc = _lib.load().tg_words_inspect(w.ctypes.data_as(_lib._u32p), w.size, C.byref(info))
     assert rc == -5 and "stack" in _lib.last_error()
 
 
