@@ -59,14 +59,15 @@ from tangerine_amd.export import Mesher  # noqa: E402
 PEAK_FP32_TFLOPS = 157.3          # MI355X published FP32 vector peak (packed issue)
 PEAK_FP32_NONPACKED_TFLOPS = 78.6  # same without packed FP32 (not used by these kernels)
 PEAK_HBM_GBS = 8000.0
+PEAK_BF16_DENSE_TFLOPS = 2500.0
 
-# HBM bytes per k_eval_blocks launch, measured with rocprofv3 --pmc FETCH_SIZE
-# and WRITE_SIZE (separate runs) on the plain exact export at 1024^3 (launch
-# 2 points per lane, 4 waves), each counter scaled by what it reads for a
-# 1 GiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).  The algorithmic
-# bytes of that launch are 4.54e9 (2,146,689 tiles x 2048 B of samples plus
-# 64 B of sign bits each); counters cannot be read inside a timed run.
-MEASURED_TRAFFIC_BYTES = {"export1024": 4.671e9}
+# HBM bytes per launch of the block-evaluation kernel (k_eval_batch<P,true>,
+# persistent), measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+# (separate runs) on the plain exact exports, each counter scaled by what it
+# reads for a 1 GiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).  At
+# 1024^3 the algorithmic bytes are 4.54e9 (2,146,689 tiles x 2048 B of samples
+# plus 64 B of sign bits each); counters cannot be read inside a timed run.
+MEASURED_TRAFFIC_BYTES = {"export1024": 4.816e9, "export512": 0.603e9}
 
 WORKLOADS = {
     "export1024": dict(seed=1, primitives=2048, lo=-8.0, hi=8.0, cells=1024),
@@ -87,6 +88,15 @@ def roofline(flops: float, ms: float, kernel: str, **extra) -> dict:
            "flops_per_launch": flops, "launch_ms": ms}
     out.update(extra)
     return out
+
+
+def block_kernel(st: dict, prune: bool) -> str:
+    """The grid-evaluation kernel an export's stats describe."""
+    if not prune:
+        return "k_eval_tiles"
+    if st["block_workgroups"]:
+        return f"k_eval_batch<{st['points_per_lane']},true> (persistent block evaluation, one tree)"
+    return f"k_eval_blocks<{st['points_per_lane']}>"
 
 
 # ---- CPU baselines -------------------------------------------------------------
@@ -306,8 +316,21 @@ def surrogate_leg(mesher, prog, wl, origin, step, cells, args) -> dict:
                 "samples": fit["samples"], "calib_samples": fit["calib_samples"], "steps": fit["steps"],
                 "margin": fit["margin"], "mean_residual": fit["mean_residual"],
                 "loss_first": float(loss[0]), "loss_last": float(loss[-1]), "clamp": args.sur_clamp,
-                "domain": [lo[0], lo[0] + (wl["hi"] - wl["lo"]) + 2 * pad]},
+                "domain": [lo[0], lo[0] + (wl["hi"] - wl["lo"]) + 2 * pad],
+                "mfma": train_mfma(fit)},
     }
+
+
+def train_mfma(fit: dict) -> dict:
+    """BF16 MFMA work of the fit's Adam steps over the replayed training
+    graphs (gradient and Adam kernels): per 32-point group, the forward
+    products 2(96*64 + 64*64)*32 and the backward dW2, dA1 and dW1 products
+    2(64*64 + 64*64 + 64*96)*32; 16,384 points per step."""
+    per_group = 2 * (96 * 64 + 64 * 64) * 32 + 2 * (64 * 64 + 64 * 64 + 64 * 96) * 32
+    flops = float(per_group) * (16384 // 32) * fit["steps"]
+    tflops = flops / (fit["train_ms"] * 1e-3) / 1e12 if fit["train_ms"] > 0 else 0.0
+    return {"flops": flops, "tflops": tflops, "peak_bf16_dense_tflops": PEAK_BF16_DENSE_TFLOPS,
+            "frac": tflops / PEAK_BF16_DENSE_TFLOPS}
 
 
 def batch_leg(args) -> dict:
@@ -453,13 +476,14 @@ def main(argv=None) -> int:
     nverts, ntris = mesher.counts()
     prune = not args.no_prune
     launch_ms = res["block_ms"] if prune and res["block_ms"] > 0 else res["eval_ms"]
-    kernel = "k_eval_blocks" if prune else "k_eval_tiles"
+    kernel = block_kernel(st, prune)
     traffic = MEASURED_TRAFFIC_BYTES.get(args.workload) if prune and not args.fast else None
     rl = roofline(st["flops"], launch_ms, kernel,
                   traffic=traffic,
                   traffic_gbs=(traffic / (launch_ms * 1e-3) / 1e9) if traffic else None,
                   hbm_peak_gbs=PEAK_HBM_GBS,
                   launch={"points_per_lane": st["points_per_lane"], "waves": st["waves"],
+                          "workgroups": st["block_workgroups"],
                           "lds_bytes": st["block_lds_bytes"], "block_max_len": st["block_max_len"],
                           "block_max_binary": st["block_max_binary"]})
     out = {
@@ -495,7 +519,7 @@ def main(argv=None) -> int:
                                       f"{args.workload} fast")
         fms = fres["block_ms"] if prune and fres["block_ms"] > 0 else fres["eval_ms"]
         out["fast"] = {"ms_per_step": fres["ms_per_step"], "samples_per_s": samples / (fres["ms_per_step"] * 1e-3),
-                       "autotune": ftune, "roofline": roofline(fst["flops"], fms, kernel + " (fast numerics)")}
+                       "autotune": ftune, "roofline": roofline(fst["flops"], fms, block_kernel(fst, prune) + " (fast numerics)")}
         # back to the exact export (and its launch shape) for the surrogate leg's judge
         if tuning:
             mesher.set_tuning(tuning["points_per_lane"], tuning["waves"], tuning.get("min_prune_len", 0),
@@ -524,7 +548,9 @@ def main(argv=None) -> int:
         out["export512"] = {"workload": "256 random primitives (seed 1), 512^3 cells over [-6,6]^3",
                             "samples": n2, "ms_per_step": r2["ms_per_step"], "export_ms_device": r2["export_ms_device"],
                             "samples_per_s": n2 / (r2["ms_per_step"] * 1e-3), "autotune": t2,
-                            "roofline": roofline(st2["flops"], ms2, kernel)}
+                            "roofline": roofline(st2["flops"], ms2, block_kernel(st2, prune),
+                                                 traffic=MEASURED_TRAFFIC_BYTES["export512"]
+                                                 if prune and not args.fast else None)}
 
     if not args.no_cpu:
         log("cpu baselines ...")

@@ -269,10 +269,16 @@ __global__ void __launch_bounds__(256) k_predict(PredictArgs p)
 
 // ---- training ----------------------------------------------------------------------
 
+// Two waves per workgroup: the weights and the gradient accumulator are
+// shared, each wave has its own activations (135 KiB of LDS, one workgroup
+// per CU), so a 16384-point batch (512 groups of 32) fills 256 CUs with
+// every wave taking one group.
+constexpr int kTrainWaves = 2;
+
 struct TrainShared
 {
 	Weights w;
-	Acts a;
+	Acts a[kTrainWaves];
 	float acc[kParams];
 	float loss;
 };
@@ -290,21 +296,21 @@ __device__ __forceinline__ float sum16(float v)
 	return v;
 }
 
-__global__ void __launch_bounds__(64) k_train_grad(TrainArgs t)
+__global__ void __launch_bounds__(64 * kTrainWaves) k_train_grad(TrainArgs t)
 {
 	extern __shared__ __align__(16) unsigned char raw[];
 	TrainShared& sh = *reinterpret_cast<TrainShared*>(raw);
-	const int lane = threadIdx.x;
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	const int r = lane & 15, q = lane >> 4;
-	load_weights(sh.w, t.net, true, lane, 64);
-	for (int i = lane; i < kParams; i += 64) sh.acc[i] = 0.0f;
-	if (lane == 0) sh.loss = 0.0f;
+	load_weights(sh.w, t.net, true, tid, 64 * kTrainWaves);
+	for (int i = tid; i < kParams; i += 64 * kTrainWaves) sh.acc[i] = 0.0f;
+	if (tid == 0) sh.loss = 0.0f;
 	__syncthreads();
 	uint64_t base = t.batch_offset;
 	if (t.batch_counter) base = (*t.batch_counter % t.batches) * uint64_t(t.batch);
 	const float scale = 2.0f / float(t.batch);
-	Acts& a = sh.a;
-	for (uint32_t gi = 0; gi < t.groups_per_block; ++gi)
+	Acts& a = sh.a[wave];
+	for (uint32_t gi = uint32_t(wave); gi < t.groups_per_block; gi += kTrainWaves)
 	{
 		const uint64_t g = uint64_t(blockIdx.x) * t.groups_per_block + gi;
 		const uint64_t s0 = base + g * kGroup;
@@ -410,20 +416,34 @@ __global__ void __launch_bounds__(64) k_train_grad(TrainArgs t)
 	}
 	__syncthreads();
 	float* out = t.partial + size_t(blockIdx.x) * kParams;
-	for (int i = lane; i < kParams; i += 64) out[i] = sh.acc[i];
-	if (lane == 0 && t.loss_partial) t.loss_partial[blockIdx.x] = sh.loss;
+	for (int i = tid; i < kParams; i += 64 * kTrainWaves) out[i] = sh.acc[i];
+	if (tid == 0 && t.loss_partial) t.loss_partial[blockIdx.x] = sh.loss;
 }
 
-// Thread per parameter: deterministic sum of the block partials, then Adam
-// with bias correction and cosine learning-rate decay to 5%.
-__global__ void __launch_bounds__(256) k_adam(AdamArgs a)
+// A workgroup per 64 parameters: wave w sums the block partials b = w mod 4
+// for its 64 parameters (lane = parameter), the four sums are added in a
+// fixed order (deterministic), then Adam with bias correction and cosine
+// learning-rate decay to 5%.  Workgroup 0 also sums the loss partials.
+constexpr int kAdamWaves = 4;
+
+__global__ void __launch_bounds__(64 * kAdamWaves) k_adam(AdamArgs a)
 {
-	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	__shared__ float part[kAdamWaves][64];
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const uint32_t i = blockIdx.x * 64 + uint32_t(lane);
 	const uint64_t step = *a.step + 1;
+	float acc[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
 	if (i < uint32_t(kParams))
 	{
-		float g = 0.0f;
-		for (uint32_t b = 0; b < a.blocks; ++b) g += a.partial[size_t(b) * kParams + i];
+		// four independent chains keep several loads in flight
+		uint32_t b = uint32_t(wave), k = 0;
+		for (; b < a.blocks; b += kAdamWaves, k = (k + 1) & 3u) acc[k] += a.partial[size_t(b) * kParams + i];
+	}
+	part[wave][lane] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+	__syncthreads();
+	if (wave == 0 && i < uint32_t(kParams))
+	{
+		const float g = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
 		const float m = a.beta1 * a.m[i] + (1.0f - a.beta1) * g;
 		const float v = a.beta2 * a.v[i] + (1.0f - a.beta2) * g * g;
 		a.m[i] = m;
@@ -434,11 +454,20 @@ __global__ void __launch_bounds__(256) k_adam(AdamArgs a)
 		const float lr = a.lr * (0.05f + 0.95f * 0.5f * (1.0f + cospif(frac)));
 		a.params[i] -= lr * (m / c1) / (sqrtf(v / c2) + a.eps);
 	}
-	if (i == 0 && a.loss_log && step - 1 < a.log_capacity)
+	if (blockIdx.x == 0 && a.loss_log && step - 1 < a.log_capacity)
 	{
+		__syncthreads();
 		float l = 0.0f;
-		for (uint32_t b = 0; b < a.blocks; ++b) l += a.loss_partial[b];
-		a.loss_log[step - 1] = l / float(a.batch);
+		for (uint32_t b = threadIdx.x; b < a.blocks; b += 64 * kAdamWaves) l += a.loss_partial[b];
+		part[wave][lane] = l;
+		__syncthreads();
+		if (threadIdx.x == 0)
+		{
+			float t = 0.0f;
+			for (int w = 0; w < kAdamWaves; ++w)
+				for (int j = 0; j < 64; ++j) t += part[w][j];
+			a.loss_log[step - 1] = t / float(a.batch);
+		}
 	}
 }
 
@@ -471,13 +500,13 @@ hipError_t launch_predict(const PredictArgs& p, hipStream_t s)
 
 hipError_t launch_train_grad(const TrainArgs& t, uint32_t blocks, hipStream_t s)
 {
-	hipLaunchKernelGGL(k_train_grad, dim3(blocks), dim3(64), sizeof(TrainShared), s, t);
+	hipLaunchKernelGGL(k_train_grad, dim3(blocks), dim3(64 * kTrainWaves), sizeof(TrainShared), s, t);
 	return hipGetLastError();
 }
 
 hipError_t launch_adam(const AdamArgs& a, hipStream_t s)
 {
-	hipLaunchKernelGGL(k_adam, dim3((kParams + 255) / 256), dim3(256), 0, s, a);
+	hipLaunchKernelGGL(k_adam, dim3((kParams + 63) / 64), dim3(64 * kAdamWaves), 0, s, a);
 	return hipGetLastError();
 }
 

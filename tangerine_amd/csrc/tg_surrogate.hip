@@ -330,7 +330,8 @@ extern "C" int tg_surrogate_fit(tg_surrogate* s, const tg_program* exact, uint64
 	float surface_sigma, uint32_t epochs, uint32_t flags, tg_surrogate_stats* stats)
 {
 	if (!s || !exact) return fail(TG_ERR_INVALID, "tg_surrogate_fit: null argument");
-	const uint32_t batch = 16384, groups_per_block = 4;
+	// two waves per workgroup, a group of 32 points each: 256 workgroups
+	const uint32_t batch = 16384, groups_per_block = 2;
 	const uint32_t blocks = batch / (mlp::kGroup * groups_per_block);
 	const uint64_t batches = samples / batch;
 	if (batches == 0 || batches >= (1u << 20) || calib_samples == 0 || epochs == 0)
