@@ -147,11 +147,12 @@ def test_non_finite_points_match_oracle(gpu, oracle, name):
 
 @pytest.mark.parametrize("scramble", [0, 1])
 def test_sqrt_all_inputs_match_sqrtf(gpu, scramble):
-    """All 2^32 float inputs: the exact build's square root (hardware root,
-    residual selection, sqrtf itself for a wave holding 0 < x < 2^-96) gives
-    the bits of the compiler's correctly rounded sqrtf, with waves of
-    consecutive bit patterns and with every wave mixing tiny, zero, negative,
-    infinite, NaN and ordinary inputs."""
+    """All 2^32 float inputs: the exact build's square root (rsq and one
+    corrected step; sqrtf itself for a wave holding an input outside
+    [2^-96, FLT_MAX]) gives the bits of the compiler's correctly rounded
+    sqrtf, with waves of consecutive bit patterns (every input of the main
+    path meets it) and with every wave mixing tiny, zero, negative, infinite,
+    NaN and ordinary inputs."""
     from tangerine_amd import _lib
     lib = _lib.load()
     n = C.c_uint64(1)
