@@ -160,6 +160,12 @@ int tg_device_copy_probe(uint64_t bytes, int repeat, double* ms_per_copy);
  * sqrtf and counts results that differ (NaNs compare equal).  scramble = 0
  * gives each wave 64 consecutive bit patterns, 1 a bijective mix of them. */
 int tg_check_sqrt(uint32_t scramble, uint64_t* mismatches);
+/* Self-check (no reference counterpart): for each divisor b, the exact
+ * build's division by a record constant (RN(1/b) once, then one multiply and
+ * two fused multiply-adds per point) against the correctly rounded a / b for
+ * every float a that takes that path; counts differing results and the
+ * values checked. */
+int tg_check_quotient(const float* divisors, uint32_t n, uint64_t* mismatches, uint64_t* checked);
 
 /* ---- tree builders (SDF::* factories; radii and half extents) -------------- */
 tg_node* tg_sphere(float radius);                                        /* sdf_evaluator.cpp:1206 */

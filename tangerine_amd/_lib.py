@@ -90,6 +90,7 @@ SIGNATURES = {
     "tg_device_synchronize": (C.c_int, []),
     "tg_device_copy_probe": (C.c_int, [_u64, C.c_int, C.POINTER(C.c_double)]),
     "tg_check_sqrt": (C.c_int, [_u32, C.POINTER(_u64)]),
+    "tg_check_quotient": (C.c_int, [C.POINTER(C.c_float), _u32, C.POINTER(_u64), C.POINTER(_u64)]),
     "tg_sphere": (_p, [_f]),
     "tg_ellipsoid": (_p, [_f, _f, _f]),
     "tg_box": (_p, [_f, _f, _f]),

@@ -410,6 +410,23 @@ extern "C" int tg_check_sqrt(uint32_t scramble, uint64_t* mismatches)
 	return TG_OK;
 }
 
+extern "C" int tg_check_quotient(const float* divisors, uint32_t n, uint64_t* mismatches, uint64_t* checked)
+{
+	if (!mismatches || !checked || (n && !divisors)) return fail(TG_ERR_INVALID, "tg_check_quotient: null argument");
+	DevBuf d;
+	int rc = d.ensure(16);
+	if (rc) return rc;
+	struct Guard { DevBuf& a; ~Guard() { a.release(); } } g{ d };
+	TG_CHECK_HIP(hipMemset(d.ptr, 0, 16), "reset");
+	for (uint32_t i = 0; i < n; ++i)
+		TG_CHECK_HIP(exact::launch_quotient_check(divisors[i], d.as<unsigned long long>(), nullptr), "quotient check");
+	unsigned long long c[2] = { 0, 0 };
+	TG_CHECK_HIP(hipMemcpy(c, d.ptr, 16, hipMemcpyDeviceToHost), "quotient check");
+	*mismatches = c[0];
+	*checked = c[1];
+	return TG_OK;
+}
+
 extern "C" int tg_eval_points(const tg_program* p, const float* xyz, uint64_t n, float* out, uint32_t flags)
 {
 	if (!p || (n && (!xyz || !out))) return fail(TG_ERR_INVALID, "tg_eval_points: null argument");

@@ -318,6 +318,7 @@ hipError_t launch_trace(const TraceArgs& a, hipStream_t s);
 hipError_t launch_gradient(const GradArgs& a, hipStream_t s);
 // all 2^32 float inputs of the exact build's square root against sqrtf
 hipError_t launch_sqrt_check(uint32_t scramble, unsigned long long* mismatches, hipStream_t s);
+hipError_t launch_quotient_check(float b, unsigned long long* counts, hipStream_t s);
 }
 namespace fast {
 hipError_t launch_points(const PointArgs& a, hipStream_t s);

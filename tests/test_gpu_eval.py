@@ -160,6 +160,47 @@ def test_sqrt_all_inputs_match_sqrtf(gpu, scramble):
     assert n.value == 0
 
 
+def _cone_divisors(spec, count):
+    """dot(Q, Q) and Q.x of the first `count` cones of a spec, in float32 as
+    SDF::Cone (tangerine/sdf_evaluator.cpp:1296-1305) and SDFMath::Cone
+    (:229-236) form them: Q = height * (radius / height, -1)."""
+    f32 = np.float32
+    out = []
+    for tok in spec:
+        if tok[0] != "cone" or len(out) >= 2 * count:
+            continue
+        r, h = f32(tok[1]), f32(tok[2])
+        qx, qy = h * (r / h), h * f32(-1.0)
+        out += [qx * qx + qy * qy, qx]
+    return out
+
+
+@pytest.mark.gpu
+def test_quotient_by_record_constant(gpu):
+    """Division by a record constant (the cone's dot(Q, Q) and Q.x, the
+    coninder's dot(K2, K2)) as RN(1/b) once and one multiply plus two fused
+    multiply-adds per point: the correctly rounded a / b for every float a on
+    that path (all 2^32 bit patterns per divisor), over the 2048-primitive
+    tree's cone divisors, log-uniform divisors over [2^-32, 2^32] of both
+    signs, and divisors at the ends of the range and of a binade."""
+    from tangerine_amd import _lib, scenes
+    f32 = np.float32
+    rng = np.random.default_rng(11)
+    divs = _cone_divisors(scenes.random_csg_spec(1, 2048), 48)
+    divs += list(f32(np.exp2(rng.uniform(-32, 32, 64))) * rng.choice([f32(-1), f32(1)], 64))
+    specials = [1.0, 2.0 ** -32, 2.0 ** 32, 1.5, 3.0, 0.1, 7.0, -1.0]
+    divs += [f32(v) for v in specials]
+    divs += [np.nextafter(f32(1), f32(2)), np.nextafter(f32(2), f32(1)), np.nextafter(f32(2.0 ** 32), f32(0))]
+    arr = np.asarray(divs, dtype=np.float32)
+    lib = _lib.load()
+    bad, taken = C.c_uint64(1), C.c_uint64(0)
+    _lib.check(lib.tg_check_quotient(arr.ctypes.data_as(_lib._fp), len(arr), C.byref(bad), C.byref(taken)),
+               "tg_check_quotient")
+    # 2^31 values per divisor: every |a| in [2^-64, 2^64) and both zeros
+    assert taken.value == len(arr) * (2 * 128 * 2 ** 23 + 2)
+    assert bad.value == 0
+
+
 # ---- grid export ---------------------------------------------------------------
 
 class Mesher:
