@@ -278,6 +278,12 @@ void tg_tracer_destroy(tg_tracer* tracer);
  * outside it run the whole program.  Default: the origin-centred cube holding
  * the eye. */
 int tg_tracer_set_domain(tg_tracer* tracer, const float lo[3], float size);
+/* Marching passes of a render: the first marches every pixel's ray at most
+ * first_budget steps, 64 rays to a wave; each later pass the rays still
+ * marching (sorted by the cell they are in) at most `budget` more, packed so
+ * that about resume_waves waves run them, until max_steps.  first_budget <= 0:
+ * one pass.  Results do not depend on these.  Default 64, 192, 65536. */
+int tg_tracer_set_passes(tg_tracer* tracer, int first_budget, int budget, uint32_t resume_waves);
 /* Renders width*height rays on the device; hit normals by central differences
  * with step normal_h (<= 0: none).  Results stay on the device until copied. */
 int tg_tracer_render(tg_tracer* tracer, const tg_program* program, const tg_camera* camera, int max_steps,

@@ -134,6 +134,7 @@ SIGNATURES = {
     "tg_tracer_create": (C.c_int, [C.POINTER(_p)]),
     "tg_tracer_destroy": (None, [_p]),
     "tg_tracer_set_domain": (C.c_int, [_p, _fp, _f]),
+    "tg_tracer_set_passes": (C.c_int, [_p, C.c_int, C.c_int, _u32]),
     "tg_tracer_render": (C.c_int, [_p, _p, C.POINTER(TgCamera), C.c_int, _f, _f, _f, _u32]),
     "tg_tracer_copy": (C.c_int, [_p, _fp, _fp, _u32p]),
     "tg_tracer_stats": (C.c_int, [_p, C.POINTER(TgTraceStats)]),

@@ -197,7 +197,15 @@ struct CertifyArgs
 
 // ---- sphere tracing -------------------------------------------------------------
 
-constexpr int kTraceLevels = 5;
+constexpr int kTraceLevels = 6;
+
+// A ray still marching at the end of a pass: what the next pass resumes from
+// (its direction and start follow from `ray`).
+struct TraceRay
+{
+	uint32_t ray, steps;
+	float t, px, py, pz;
+};
 
 struct TraceLevel
 {
@@ -235,6 +243,21 @@ struct TraceArgs
 	uint8_t* hit_out;                 // or null
 	unsigned long long* work;         // [0] evaluations, [1] cost units, [2] hits
 	uint32_t slots;
+
+	// Passes: a pass marches each of its rays at most `budget` steps (<= 0:
+	// to max_steps) and appends the rays still marching to q_out with their
+	// cell keys (trace_key) in q_keys (q_out null: every ray ends in this
+	// pass).  The first pass takes its rays from the camera tiles or the
+	// arrays; a resumed pass takes q_in[q_order[i]] for i < q_count,
+	// q_per_wave (1..64) to a wave.
+	int budget;
+	const TraceRay* q_in;
+	const uint32_t* q_order;
+	uint32_t q_count;
+	uint32_t q_per_wave;
+	TraceRay* q_out;
+	uint32_t* q_keys;
+	uint32_t* q_out_count;
 };
 
 // Point-cloud export: cells of a regular point grid kept near the surface,
@@ -315,6 +338,8 @@ hipError_t launch_tiles(const GridArgs& a, int p, hipStream_t s);
 hipError_t launch_prune_cells(const PruneArgs& a, hipStream_t s);
 hipError_t launch_eval_blocks(const BlockEvalArgs& a, int p, hipStream_t s);
 hipError_t launch_trace(const TraceArgs& a, hipStream_t s);
+// resumed pass (a.q_in set): a wave per a.q_per_wave of the a.q_count rays
+hipError_t launch_trace_resume(const TraceArgs& a, hipStream_t s);
 hipError_t launch_gradient(const GradArgs& a, hipStream_t s);
 // all 2^32 float inputs of the exact build's square root against sqrtf
 hipError_t launch_sqrt_check(uint32_t scramble, unsigned long long* mismatches, hipStream_t s);
@@ -333,6 +358,8 @@ hipError_t launch_tiles(const GridArgs& a, int p, hipStream_t s);
 hipError_t launch_prune_cells(const PruneArgs& a, hipStream_t s);
 hipError_t launch_eval_blocks(const BlockEvalArgs& a, int p, hipStream_t s);
 hipError_t launch_trace(const TraceArgs& a, hipStream_t s);
+// resumed pass (a.q_in set): a wave per a.q_per_wave of the a.q_count rays
+hipError_t launch_trace_resume(const TraceArgs& a, hipStream_t s);
 hipError_t launch_gradient(const GradArgs& a, hipStream_t s);
 }
 

@@ -72,6 +72,12 @@ class Tracer:
         except Exception:
             pass
 
+    def set_passes(self, first_budget: int, budget: int, resume_waves: int = 65536) -> None:
+        """Steps of the first marching pass and of each later one, and the waves
+        a later pass spreads its rays over (first_budget <= 0: one pass)."""
+        check(self.lib.tg_tracer_set_passes(self.h, int(first_budget), int(budget), int(resume_waves)),
+              "tg_tracer_set_passes")
+
     def set_domain(self, lo, size: float) -> None:
         check(self.lib.tg_tracer_set_domain(self.h, _farr(lo), float(size)), "tg_tracer_set_domain")
 

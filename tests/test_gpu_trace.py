@@ -68,12 +68,14 @@ def test_gradients(gpu, oracle, name):
     assert np.array_equal(bits(cen), bits(oracle.gradient_central(g["words"], g["grad_points"], 1e-3)))
 
 
-def _render_case(oracle, spec, eye, width, height, prune, fast=False):
+def _render_case(oracle, spec, eye, width, height, prune, fast=False, passes=None):
     from tangerine_amd import scenes, trace
     node = scenes.build_sdf(spec)
     prog = node.program()
     cam = trace.look_at(eye, (0, 0, 0), (0, 0, 1), 60.0, width, height)
     tr = trace.Tracer()
+    if passes is not None:
+        tr.set_passes(*passes)
     st = tr.render(prog, cam, 256, 1e-4, 40.0, 1e-3, fast=fast, prune=prune)
     t, n, steps = tr.image()
     tr.close()
@@ -103,6 +105,24 @@ def test_render_bit_identical_to_oracle(gpu, oracle, prune):
     assert np.array_equal(bits(n[hits]), bits(on))
     assert np.all(n[~hits] == 0)
     assert st["hits"] == hits.sum() and st["rays"] == 160 * 90
+
+
+@pytest.mark.parametrize("fast", [False, True])
+def test_render_passes_give_the_same_image(gpu, oracle, fast):
+    """Marching in passes (the rays still marching after a pass's step budget
+    sorted by cell and packed anywhere from 1 to 64 to a wave for the next
+    pass) changes no bit: one pass, one step per pass (256 passes), and
+    uneven splits give the default split's t, step counts and normals, and
+    the same work counts."""
+    from tangerine_amd import scenes, trace
+    spec = scenes.random_csg_spec(1, 256)
+    eye = trace.seeded_eye(3)
+    ref = _render_case(oracle, spec, eye, 160, 90, True, fast)
+    for passes in [(0, 0, 1), (1, 1, 8192), (3, 7, 100), (255, 1, 8192), (40, 200, 1), (16, 16, 1 << 24)]:
+        got = _render_case(oracle, spec, eye, 160, 90, True, fast, passes)
+        for a, b in zip(ref[4:], got[4:]):
+            assert np.array_equal(bits(a), bits(b)), passes
+        assert got[3]["hits"] == ref[3]["hits"] and got[3]["evaluations"] == ref[3]["evaluations"], passes
 
 
 def test_render_full_hd_sampled(gpu, oracle):
