@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <cstdint>
 #include <string>
 #include <vector>
 
@@ -81,6 +82,27 @@ struct CellPrograms
 
 int prune_cells(hipStream_t stream, const tg_program* p, const float origin[3], const float cell[3], float radius,
 	const uint32_t counts[3], bool fast, CellPrograms& out);
+
+// A cell grid to prune a program over.
+struct CellGrid
+{
+	float origin[3], cell[3], radius;
+	uint32_t counts[3];
+};
+
+// Job tables of prune_cells_many (kept between calls: the upload reads `host`).
+struct PruneLaunches
+{
+	DevBuf dev;
+	std::vector<uint8_t> host;
+};
+
+// prune_cells_start for n programs on one stream: every program's height-h
+// pass in one launch per height, then every emission in one launch (instead
+// of a launch per program and height).  Each is then finished with
+// prune_cells_finish on the same stream.
+int prune_cells_many(hipStream_t stream, const tg_program* const* progs, const CellGrid* grids, uint32_t n, bool fast,
+	CellPrograms* const* outs, PruneLaunches& work);
 // The same in two halves: start issues the passes on `stream` without waiting;
 // finish waits for the stream, reruns the emission if the pool overflowed and
 // fills the counts (same arguments).  Between them the caller may issue other

@@ -1605,6 +1605,8 @@ struct tg_batch
 	std::vector<tg_grid> grids;
 	DevBuf trees, items, head, work;
 	DevBuf tables;                // per-workgroup decision tables and tile pools
+	PruneLaunches prune;          // every tree's level pruning and emission as shared launches
+	DevBuf queue_sort;            // the queue's keys, values, tree offsets and sort storage
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
 	BatchArgs args = {};
@@ -1717,7 +1719,7 @@ extern "C" void tg_batch_destroy(tg_batch* b)
 	if (b->stream) (void)hipStreamSynchronize(b->stream);
 	drop_graph(b);
 	for (tg_mesher* m : b->meshers) destroy_mesher(m);
-	for (DevBuf* d : { &b->trees, &b->items, &b->head, &b->work, &b->tables })
+	for (DevBuf* d : { &b->trees, &b->items, &b->head, &b->work, &b->tables, &b->prune.dev, &b->queue_sort })
 		d->release();
 	for (hipEvent_t e : b->ev)
 		if (e) (void)hipEventDestroy(e);
@@ -1778,20 +1780,35 @@ extern "C" int tg_batch_export(tg_batch* b, const tg_program* const* programs, c
 	std::vector<BlockEvalArgs> args(count);
 	BatchArgs ba{};
 	uint64_t samples = 0;
-	// every tree's pruning is issued on its own stream before any is waited for
+	// every tree's pruning on the batch's stream: each height of every tree's
+	// level pass in one launch, every tree's emission in one launch, one wait
+	const bool fast = (flags & TG_FLAG_FAST) != 0;
+	std::vector<CellGrid> cgs(count);
+	std::vector<CellPrograms*> outs(count);
 	for (uint32_t i = 0; i < count; ++i)
 	{
 		tg_mesher* m = b->meshers[i];
 		m->have_grid = false;
 		m->nbricks = 0;
 		rc = layout(m, &grids[i]);
-		if (!rc) rc = prune_blocks_start(m, programs[i], &grids[i], flags);
 		if (rc) return rc;
+		const BlockGrid bg = block_grid(m, &grids[i]);
+		CellGrid& cg = cgs[i];
+		for (int k = 0; k < 3; ++k)
+		{
+			cg.origin[k] = bg.origin[k];
+			cg.cell[k] = bg.cell[k];
+			cg.counts[k] = bg.counts[k];
+		}
+		cg.radius = bg.radius;
+		outs[i] = &m->blocks;
 	}
+	rc = prune_cells_many(b->stream, programs, cgs.data(), count, fast, outs.data(), b->prune);
+	if (rc) return rc;
 	for (uint32_t i = 0; i < count; ++i)
 	{
 		tg_mesher* m = b->meshers[i];
-		rc = prune_blocks_finish(m, programs[i], &grids[i], flags);
+		rc = prune_cells_finish(b->stream, programs[i], cgs[i].origin, cgs[i].cell, cgs[i].counts, fast, m->blocks);
 		if (rc) return rc;
 		if (!m->blocks.valid) return fail(TG_ERR_LIMIT, "tg_batch_export: a tree could not be pruned");
 		args[i] = block_args(m, programs[i], &grids[i], nullptr, nullptr, 0, b->work.as<unsigned long long>() + 2 * i);
@@ -1800,43 +1817,39 @@ extern "C" int tg_batch_export(tg_batch* b, const tg_program* const* programs, c
 		ba.max_len = std::max(ba.max_len, args[i].max_len);
 		samples += uint64_t(m->args.nx) * m->args.ny * m->args.nz;
 	}
-	// 2. the queue: every tree's blocks, the most expensive block program
-	// first (program cost x tiles of the block)
-	struct Item
-	{
-		double cost;
-		uint32_t tree, block;
-	};
-	std::vector<Item> order;
-	std::vector<std::vector<BlockDesc>> desc(count);
+	// 2. the queue, sorted on the device: every tree's blocks, the most
+	// expensive block program first
+	std::vector<uint32_t> first(count + 1, 0);
 	for (uint32_t i = 0; i < count; ++i)
 	{
-		desc[i].resize(args[i].nblocks);
-		TG_CHECK_HIP(hipMemcpyAsync(desc[i].data(), args[i].desc, desc[i].size() * sizeof(BlockDesc),
-			hipMemcpyDeviceToHost, b->meshers[i]->stream), "block descriptors");
+		// queue items pack tree << 20 | block (count <= 4096 trees)
+		if (args[i].nblocks >= (1u << 20)) return fail(TG_ERR_LIMIT, "tg_batch_export: a tree has 2^20 blocks or more");
+		first[i + 1] = first[i] + args[i].nblocks;
 	}
-	for (uint32_t i = 0; i < count; ++i)
-	{
-		TG_CHECK_HIP(hipStreamSynchronize(b->meshers[i]->stream), "block descriptors");
-		for (uint32_t k = 0; k < args[i].nblocks; ++k) order.push_back(Item{ double(desc[i][k].flops), i, k });
-	}
-	std::stable_sort(order.begin(), order.end(), [](const Item& x, const Item& y) { return x.cost > y.cost; });
-	std::vector<uint32_t> items(2 * order.size());
-	for (size_t k = 0; k < order.size(); ++k)
-	{
-		items[2 * k] = order[k].tree;
-		items[2 * k + 1] = order[k].block;
-	}
-	rc = b->items.ensure(std::max<size_t>(items.size(), 2) * 4);
+	const uint32_t total = first[count];
+	if (total >= (1u << 30)) return fail(TG_ERR_LIMIT, "tg_batch_export: too many blocks");
+	size_t sort_tmp = 0;
+	TG_CHECK_HIP(batch_queue_by_cost(nullptr, nullptr, count, total, nullptr, nullptr, nullptr, sort_tmp, b->stream),
+		"queue sort sizing");
+	rc = b->items.ensure(std::max<size_t>(2 * size_t(total), 2) * 4);
+	if (!rc) rc = b->queue_sort.ensure(std::max<size_t>(4 * size_t(total), 1) * 4 + 256 + std::max<size_t>(sort_tmp, 256) +
+	                                  (size_t(count) + 1) * 4);
 	if (rc) return rc;
-	TG_CHECK_HIP(hipMemcpy(b->trees.ptr, args.data(), size_t(count) * sizeof(BlockEvalArgs), hipMemcpyHostToDevice),
+	uint32_t* sort_buf = b->queue_sort.as<uint32_t>();
+	uint32_t* first_dev = sort_buf + 4 * size_t(total);
+	void* sort_temp = reinterpret_cast<unsigned char*>(b->queue_sort.ptr) +
+	                  ((4 * size_t(total) + count + 1) * 4 + 255) / 256 * 256;
+	TG_CHECK_HIP(hipMemcpyAsync(b->trees.ptr, args.data(), size_t(count) * sizeof(BlockEvalArgs), hipMemcpyHostToDevice,
+		b->stream), "upload");
+	TG_CHECK_HIP(hipMemcpyAsync(first_dev, first.data(), (size_t(count) + 1) * 4, hipMemcpyHostToDevice, b->stream),
 		"upload");
-	if (!items.empty())
-		TG_CHECK_HIP(hipMemcpy(b->items.ptr, items.data(), items.size() * 4, hipMemcpyHostToDevice), "upload");
+	TG_CHECK_HIP(batch_queue_by_cost(b->trees.as<const BlockEvalArgs>(), first_dev, count, total, sort_buf,
+		b->items.as<uint32_t>(), sort_temp, sort_tmp, b->stream), "queue sort");
+	TG_CHECK_HIP(hipStreamSynchronize(b->stream), "queue sort");
 	ba.trees = b->trees.as<const BlockEvalArgs>();
 	ba.items = b->items.as<const uint32_t>();
 	ba.ntrees = count;
-	ba.total = uint32_t(order.size());
+	ba.total = total;
 	ba.head = b->head.as<unsigned int>();
 	b->args = ba;
 	auto t1 = std::chrono::steady_clock::now();

@@ -328,6 +328,12 @@ struct GradArgs
 namespace exact {
 hipError_t launch_points(const PointArgs& a, hipStream_t s);
 hipError_t launch_prune_level(const LevelArgs& a, hipStream_t s);
+// many programs' passes in one launch: job k owns waves [wave_base[k], wave_base[k + 1]) of `waves`
+hipError_t launch_prune_level_multi(const LevelArgs* jobs, const uint32_t* wave_base, uint32_t njobs, uint32_t waves,
+	hipStream_t s);
+// many programs' emissions in one launch: job k owns blocks [block_base[k], ...) of `blocks`
+hipError_t launch_prune_cells_multi(const PruneArgs* jobs, const uint32_t* block_base, uint32_t njobs, uint32_t blocks,
+	uint32_t max_slots, hipStream_t s);
 hipError_t launch_vox(const VoxArgs& a, int stage, hipStream_t s);  // 0 select, 1 list
 hipError_t launch_pivot(const PivotArgs& a, hipStream_t s);
 hipError_t launch_cloud(const CloudArgs& a, int stage, hipStream_t s);  // 0 select, 1 points, 2 refine
@@ -348,6 +354,12 @@ hipError_t launch_quotient_check(float b, unsigned long long* counts, hipStream_
 namespace fast {
 hipError_t launch_points(const PointArgs& a, hipStream_t s);
 hipError_t launch_prune_level(const LevelArgs& a, hipStream_t s);
+// many programs' passes in one launch: job k owns waves [wave_base[k], wave_base[k + 1]) of `waves`
+hipError_t launch_prune_level_multi(const LevelArgs* jobs, const uint32_t* wave_base, uint32_t njobs, uint32_t waves,
+	hipStream_t s);
+// many programs' emissions in one launch: job k owns blocks [block_base[k], ...) of `blocks`
+hipError_t launch_prune_cells_multi(const PruneArgs* jobs, const uint32_t* block_base, uint32_t njobs, uint32_t blocks,
+	uint32_t max_slots, hipStream_t s);
 hipError_t launch_vox(const VoxArgs& a, int stage, hipStream_t s);  // 0 select, 1 list
 hipError_t launch_pivot(const PivotArgs& a, hipStream_t s);
 hipError_t launch_cloud(const CloudArgs& a, int stage, hipStream_t s);  // 0 select, 1 points, 2 refine

@@ -46,6 +46,12 @@ hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, voi
 // Blocks (all n, or the n of `list`) ordered by their pruned program's cost,
 // most expensive first, into `order`; keys needs 2n words, values n.  With
 // temp = null only sizes temp_bytes.
+// The batch queue: every tree's blocks (tree t owns items [first[t],
+// first[t + 1])), most expensive block program first (stable), written to
+// `items` as (tree, block) word pairs; sort_buf holds 4 total words.  With
+// temp null, sets temp_bytes only.
+hipError_t batch_queue_by_cost(const BlockEvalArgs* trees, const uint32_t* first, uint32_t ntrees, uint32_t total,
+	uint32_t* sort_buf, uint32_t* items, void* temp, size_t& temp_bytes, hipStream_t s);
 hipError_t blocks_by_cost(const BlockDesc* desc, const uint32_t* list, uint32_t n, uint32_t* keys,
 	uint32_t* values, uint32_t* order, void* temp, size_t& temp_bytes, hipStream_t s);
 

@@ -401,6 +401,62 @@ __global__ void __launch_bounds__(256) k_block_costs(const BlockDesc* desc, cons
 
 }  // namespace
 
+namespace {
+
+// Thread per item of a batch queue: tree t owns items [first[t], first[t + 1]);
+// key = its block program's cost, value = tree << 20 | block.
+__global__ void __launch_bounds__(256) k_batch_keys(const BlockEvalArgs* trees, const uint32_t* first, uint32_t ntrees,
+	uint32_t total, uint32_t* keys, uint32_t* values)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= total) return;
+	uint32_t lo = 0, hi = ntrees;
+	while (hi - lo > 1)
+	{
+		const uint32_t mid = (lo + hi) / 2;
+		if (first[mid] <= i)
+			lo = mid;
+		else
+			hi = mid;
+	}
+	const uint32_t b = i - first[lo];
+	keys[i] = trees[lo].desc[b].flops;
+	values[i] = (lo << 20) | b;
+}
+
+// The sorted values as (tree, block) word pairs.
+__global__ void __launch_bounds__(256) k_batch_items(const uint32_t* values, uint32_t total, uint32_t* items)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= total) return;
+	items[2 * size_t(i)] = values[i] >> 20;
+	items[2 * size_t(i) + 1] = values[i] & 0xfffffu;
+}
+
+}  // namespace
+
+hipError_t batch_queue_by_cost(const BlockEvalArgs* trees, const uint32_t* first, uint32_t ntrees, uint32_t total,
+	uint32_t* sort_buf, uint32_t* items, void* temp, size_t& temp_bytes, hipStream_t s)
+{
+	uint32_t* keys = sort_buf;
+	uint32_t* keys_out = sort_buf + total;
+	uint32_t* values = sort_buf + 2 * size_t(total);
+	uint32_t* values_out = sort_buf + 3 * size_t(total);
+	if (!temp)
+		return hipcub::DeviceRadixSort::SortPairsDescending(temp, temp_bytes, keys, keys_out, values, values_out,
+			int(total), 0, 32, s);
+	if (total == 0) return hipSuccess;
+	hipLaunchKernelGGL(k_batch_keys, dim3((total + 255) / 256), dim3(256), 0, s, trees, first, ntrees, total, keys,
+		values);
+	hipError_t e = hipGetLastError();
+	if (e != hipSuccess) return e;
+	e = hipcub::DeviceRadixSort::SortPairsDescending(temp, temp_bytes, keys, keys_out, values, values_out, int(total),
+		0, 32, s);
+	if (e != hipSuccess) return e;
+	hipLaunchKernelGGL(k_batch_items, dim3((total + 255) / 256), dim3(256), 0, s, values_out, total, items);
+	return hipGetLastError();
+}
+
 hipError_t blocks_by_cost(const BlockDesc* desc, const uint32_t* list, uint32_t n, uint32_t* keys,
 	uint32_t* values, uint32_t* order, void* temp, size_t& temp_bytes, hipStream_t s)
 {

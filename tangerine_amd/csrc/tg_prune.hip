@@ -4,11 +4,19 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "../../include/tangerine_gpu.h"
 #include "tg_capi_internal.h"
 #include "tg_kernels.h"
+
+#define TG_CHECK_HIP(expr, what)                         \
+	do                                                   \
+	{                                                    \
+		hipError_t _e = (expr);                          \
+		if (_e != hipSuccess) return hip_fail(_e, what); \
+	} while (0)
 
 namespace tg {
 
@@ -54,15 +62,12 @@ namespace {
 // One emission pass (and, on the first, the level-parallel decision passes)
 // with the pool capacity out.state.capacity; the five counters are copied to
 // out.host (pinned) on `stream`.
-int prune_pass(hipStream_t stream, const tg_program* p, const float origin[3], const float cell[3],
-	const uint32_t counts[3], bool fast, CellPrograms& out, bool first)
+// Emission arguments of a prepared pruning (pool capacity: state.capacity).
+PruneArgs prune_args(const tg_program* p, const float origin[3], const float cell[3], const uint32_t counts[3],
+	const CellPrograms& out)
 {
 	const DecodedProgram& d = p->decoded;
-	PruneState& ps = out.state;
-	int rc = out.pool.ensure(ps.capacity * sizeof(Entry));
-	if (rc) return rc;
-	hipError_t e = hipMemsetAsync(out.counters.ptr, 0, 64, stream);
-	if (e != hipSuccess) return hip_fail(e, "prune counters");
+	const PruneState& ps = out.state;
 	PruneArgs pa{};
 	pa.prog = static_cast<const Instr*>(p->device.ptr);
 	pa.entries = p->entries.as<const Entry>();
@@ -80,48 +85,78 @@ int prune_pass(hipStream_t stream, const tg_program* p, const float origin[3], c
 	pa.counters = out.counters.as<unsigned long long>();
 	pa.scratch = out.scratch.as<uint64_t>();
 	pa.cells_per_wave = ps.cells_per_wave;
-	const uint64_t waves = ps.waves;
-	if (first && ps.level_parallel)
-	{
-		LevelArgs la{};
-		la.prog = pa.prog;
-		la.entries = pa.entries;
-		la.ngroups = uint32_t(waves);
-		la.cells_per_wave = ps.cells_per_wave;
-		la.ox = pa.ox; la.oy = pa.oy; la.oz = pa.oz;
-		la.cx = pa.cx; la.cy = pa.cy; la.cz = pa.cz;
-		la.radius = ps.decide_radius;
-		la.nx = pa.nx; la.ny = pa.ny; la.nz = pa.nz;
-		la.values = out.values.as<float>();
-		la.scratch = pa.scratch;
-		la.n_binary = pa.n_binary;
-		la.n_entries = pa.n_entries;
-		la.counts = out.lens.as<uint32_t>();
-		e = hipMemsetAsync(out.lens.ptr, 0, waves * 64 * 4, stream);
-		if (e != hipSuccess) return hip_fail(e, "length reset");
-		const size_t heights = ps.heights;
-		// values and decisions upward from height 1 (k_prune_level evaluates
-		// leaf operands in place), then liveness and lengths downward
-		for (size_t step = 1; step < 2 * heights; ++step)
-		{
-			const size_t h = step < heights ? step : 2 * heights - 1 - step;
-			la.liveness = step < heights ? 0 : 1;
-			la.list = out.levels.as<const uint32_t>() + ps.level_first[h];
-			la.count = ps.level_first[h + 1] - ps.level_first[h];
-			// a few cell groups per wave while enough waves remain to fill the GPU
-			la.groups_per_wave = 1;
-			while (la.groups_per_wave < 8 && uint64_t(la.count) * waves / (la.groups_per_wave * 2) >= 8192)
-				la.groups_per_wave *= 2;
-			e = fast ? fast::launch_prune_level(la, stream) : exact::launch_prune_level(la, stream);
-			if (e != hipSuccess) return hip_fail(e, "level pruning launch");
-		}
-	}
 	// masks, liveness and lengths stay in scratch for the rerun after a pool overflow
 	pa.masks_ready = ps.level_parallel ? 1 : 0;
 	pa.counts = out.lens.as<const uint32_t>();
 	// emission only: four waves per cell group while that leaves under ~8
 	// waves per SIMD
-	pa.splits = (ps.level_parallel && waves * 4 <= 8192) ? 4u : 1u;
+	pa.splits = (ps.level_parallel && ps.waves * 4 <= 8192) ? 4u : 1u;
+	return pa;
+}
+
+// Level-parallel pass arguments (without the height's list) of a prepared pruning.
+LevelArgs level_args(const PruneArgs& pa, const CellPrograms& out)
+{
+	const PruneState& ps = out.state;
+	LevelArgs la{};
+	la.prog = pa.prog;
+	la.entries = pa.entries;
+	la.ngroups = uint32_t(ps.waves);
+	la.cells_per_wave = ps.cells_per_wave;
+	la.ox = pa.ox; la.oy = pa.oy; la.oz = pa.oz;
+	la.cx = pa.cx; la.cy = pa.cy; la.cz = pa.cz;
+	la.radius = ps.decide_radius;
+	la.nx = pa.nx; la.ny = pa.ny; la.nz = pa.nz;
+	la.values = out.values.as<float>();
+	la.scratch = pa.scratch;
+	la.n_binary = pa.n_binary;
+	la.n_entries = pa.n_entries;
+	la.counts = out.lens.as<uint32_t>();
+	return la;
+}
+
+// Height step `step` (1 .. 2 heights - 1) of the level-parallel pass: values
+// and decisions upward from height 1 (leaf operands are evaluated in place),
+// then liveness and lengths downward.
+void level_step(LevelArgs& la, const CellPrograms& out, size_t step)
+{
+	const PruneState& ps = out.state;
+	const size_t heights = ps.heights;
+	const size_t h = step < heights ? step : 2 * heights - 1 - step;
+	la.liveness = step < heights ? 0 : 1;
+	la.list = out.levels.as<const uint32_t>() + ps.level_first[h];
+	la.count = ps.level_first[h + 1] - ps.level_first[h];
+	// a few cell groups per wave while enough waves remain to fill the GPU
+	la.groups_per_wave = 1;
+	while (la.groups_per_wave < 8 && uint64_t(la.count) * ps.waves / (la.groups_per_wave * 2) >= 8192)
+		la.groups_per_wave *= 2;
+}
+
+// One emission pass (and, on the first, the level-parallel decision passes)
+// with the pool capacity out.state.capacity; the five counters are copied to
+// out.host (pinned) on `stream`.
+int prune_pass(hipStream_t stream, const tg_program* p, const float origin[3], const float cell[3],
+	const uint32_t counts[3], bool fast, CellPrograms& out, bool first)
+{
+	PruneState& ps = out.state;
+	int rc = out.pool.ensure(ps.capacity * sizeof(Entry));
+	if (rc) return rc;
+	hipError_t e = hipMemsetAsync(out.counters.ptr, 0, 64, stream);
+	if (e != hipSuccess) return hip_fail(e, "prune counters");
+	const PruneArgs pa = prune_args(p, origin, cell, counts, out);
+	const uint64_t waves = ps.waves;
+	if (first && ps.level_parallel)
+	{
+		LevelArgs la = level_args(pa, out);
+		e = hipMemsetAsync(out.lens.ptr, 0, waves * 64 * 4, stream);
+		if (e != hipSuccess) return hip_fail(e, "length reset");
+		for (size_t step = 1; step < 2 * ps.heights; ++step)
+		{
+			level_step(la, out, step);
+			e = fast ? fast::launch_prune_level(la, stream) : exact::launch_prune_level(la, stream);
+			if (e != hipSuccess) return hip_fail(e, "level pruning launch");
+		}
+	}
 	e = fast ? fast::launch_prune_cells(pa, stream) : exact::launch_prune_cells(pa, stream);
 	if (e != hipSuccess) return hip_fail(e, "cell pruning launch");
 	e = hipMemcpyAsync(out.host, out.counters.ptr, 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream);
@@ -131,10 +166,16 @@ int prune_pass(hipStream_t stream, const tg_program* p, const float origin[3], c
 
 }  // namespace
 
-int prune_cells_start(hipStream_t stream, const tg_program* p, const float origin[3], const float cell[3],
-	float radius, const uint32_t counts[3], bool fast, CellPrograms& out)
+namespace {
+
+// Everything of prune_cells_start before the passes: sizes, buffers and the
+// upload of the entries by height.  *ready is false when the program is not
+// pruned here.
+int prune_prepare(hipStream_t stream, const tg_program* p, const float origin[3], const float cell[3],
+	float radius, const uint32_t counts[3], CellPrograms& out, bool* ready)
 {
 	const DecodedProgram& d = p->decoded;
+	*ready = false;
 	out.valid = false;
 	out.state.pending = false;
 	if (!d.prunable() || d.entries.empty() || d.entries.size() >= 65535) return TG_OK;
@@ -191,9 +232,139 @@ int prune_cells_start(hipStream_t stream, const tg_program* p, const float origi
 	// to the exact total and rerun if any cell did not fit (prune_cells_finish).
 	ps.capacity = std::max<uint64_t>(out.pool.bytes / sizeof(Entry),
 		std::min<uint64_t>(ncells * std::max<size_t>(d.entries.size() / 4, 16), (1ull << 30) / sizeof(Entry)));
+	*ready = true;
+	return TG_OK;
+}
+
+}  // namespace
+
+int prune_cells_start(hipStream_t stream, const tg_program* p, const float origin[3], const float cell[3],
+	float radius, const uint32_t counts[3], bool fast, CellPrograms& out)
+{
+	bool ready = false;
+	int rc = prune_prepare(stream, p, origin, cell, radius, counts, out, &ready);
+	if (rc || !ready) return rc;
 	rc = prune_pass(stream, p, origin, cell, counts, fast, out, true);
 	if (rc) return rc;
-	ps.pending = true;
+	out.state.pending = true;
+	return TG_OK;
+}
+
+int prune_cells_many(hipStream_t stream, const tg_program* const* progs, const CellGrid* grids, uint32_t n, bool fast,
+	CellPrograms* const* outs, PruneLaunches& work)
+{
+	std::vector<uint8_t> ready(n, 0);
+	std::vector<PruneArgs> pas(n);
+	size_t steps = 0;
+	for (uint32_t i = 0; i < n; ++i)
+	{
+		bool r = false;
+		int rc = prune_prepare(stream, progs[i], grids[i].origin, grids[i].cell, grids[i].radius, grids[i].counts,
+			*outs[i], &r);
+		if (rc) return rc;
+		if (!r) continue;
+		CellPrograms& out = *outs[i];
+		ready[i] = 1;
+		rc = out.pool.ensure(out.state.capacity * sizeof(Entry));
+		if (rc) return rc;
+		TG_CHECK_HIP(hipMemsetAsync(out.counters.ptr, 0, 64, stream), "prune counters");
+		if (out.state.level_parallel)
+		{
+			TG_CHECK_HIP(hipMemsetAsync(out.lens.ptr, 0, out.state.waves * 64 * 4, stream), "length reset");
+			steps = std::max(steps, 2 * out.state.heights);
+		}
+		pas[i] = prune_args(progs[i], grids[i].origin, grids[i].cell, grids[i].counts, out);
+	}
+	// host blob: per height step, its level jobs then their wave bases; then
+	// the emission jobs and their block bases
+	std::vector<uint8_t>& blob = work.host;
+	blob.clear();
+	auto put = [&](const void* src, size_t bytes) {
+		const size_t at = blob.size();
+		blob.resize(at + ((bytes + 15) & ~size_t(15)));
+		std::memcpy(blob.data() + at, src, bytes);
+		return at;
+	};
+	struct Launch
+	{
+		size_t jobs, bases;
+		uint32_t njobs, units;
+	};
+	std::vector<Launch> level_launches;
+	for (size_t step = 1; step < steps; ++step)
+	{
+		std::vector<LevelArgs> jobs;
+		std::vector<uint32_t> base;
+		uint32_t waves = 0;
+		for (uint32_t i = 0; i < n; ++i)
+		{
+			const CellPrograms& out = *outs[i];
+			if (!ready[i] || !out.state.level_parallel || step >= 2 * out.state.heights) continue;
+			LevelArgs la = level_args(pas[i], out);
+			level_step(la, out, step);
+			const uint32_t per_entry = (la.ngroups + la.groups_per_wave - 1) / la.groups_per_wave;
+			base.push_back(waves);
+			waves += la.count * per_entry;
+			jobs.push_back(la);
+		}
+		if (jobs.empty()) continue;
+		const size_t j = put(jobs.data(), jobs.size() * sizeof(LevelArgs));
+		const size_t b = put(base.data(), base.size() * 4);
+		level_launches.push_back(Launch{ j, b, uint32_t(jobs.size()), waves });
+	}
+	std::vector<PruneArgs> ejobs;
+	std::vector<uint32_t> ebase;
+	uint32_t blocks = 0, max_slots = 1;
+	for (uint32_t i = 0; i < n; ++i)
+	{
+		if (!ready[i]) continue;
+		const PruneArgs& pa = pas[i];
+		const uint32_t ncells = pa.nx * pa.ny * pa.nz;
+		const uint32_t splits = pa.masks_ready ? pa.splits : 1u;
+		ebase.push_back(blocks);
+		blocks += (ncells + pa.cells_per_wave - 1) / pa.cells_per_wave * splits;
+		max_slots = std::max(max_slots, pa.slots);
+		ejobs.push_back(pa);
+	}
+	size_t ej = 0, eb = 0;
+	if (!ejobs.empty())
+	{
+		ej = put(ejobs.data(), ejobs.size() * sizeof(PruneArgs));
+		eb = put(ebase.data(), ebase.size() * 4);
+	}
+	if (!blob.empty())
+	{
+		int rc = work.dev.ensure(blob.size());
+		if (rc) return rc;
+		TG_CHECK_HIP(hipMemcpyAsync(work.dev.ptr, blob.data(), blob.size(), hipMemcpyHostToDevice, stream),
+			"pruning job upload");
+	}
+	const uint8_t* dev = work.dev.as<const uint8_t>();
+	for (const Launch& l : level_launches)
+	{
+		const LevelArgs* jobs = reinterpret_cast<const LevelArgs*>(dev + l.jobs);
+		const uint32_t* base = reinterpret_cast<const uint32_t*>(dev + l.bases);
+		const hipError_t e = fast ? fast::launch_prune_level_multi(jobs, base, l.njobs, l.units, stream)
+		                          : exact::launch_prune_level_multi(jobs, base, l.njobs, l.units, stream);
+		if (e != hipSuccess) return hip_fail(e, "level pruning launch");
+	}
+	if (!ejobs.empty())
+	{
+		const PruneArgs* jobs = reinterpret_cast<const PruneArgs*>(dev + ej);
+		const uint32_t* base = reinterpret_cast<const uint32_t*>(dev + eb);
+		const hipError_t e =
+			fast ? fast::launch_prune_cells_multi(jobs, base, uint32_t(ejobs.size()), blocks, max_slots, stream)
+			     : exact::launch_prune_cells_multi(jobs, base, uint32_t(ejobs.size()), blocks, max_slots, stream);
+		if (e != hipSuccess) return hip_fail(e, "cell pruning launch");
+	}
+	for (uint32_t i = 0; i < n; ++i)
+	{
+		if (!ready[i]) continue;
+		CellPrograms& out = *outs[i];
+		TG_CHECK_HIP(hipMemcpyAsync(out.host, out.counters.ptr, 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+			stream), "cell pruning counters");
+		out.state.pending = true;
+	}
 	return TG_OK;
 }
 
