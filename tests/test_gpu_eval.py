@@ -196,8 +196,8 @@ def test_quotient_by_record_constant(gpu):
     bad, taken = C.c_uint64(1), C.c_uint64(0)
     _lib.check(lib.tg_check_quotient(arr.ctypes.data_as(_lib._fp), len(arr), C.byref(bad), C.byref(taken)),
                "tg_check_quotient")
-    # 2^31 values per divisor: every |a| in [2^-64, 2^64) and both zeros
-    assert taken.value == len(arr) * (2 * 128 * 2 ** 23 + 2)
+    # 2^31 values per divisor: every |a| in [2^-64, 2^64)
+    assert taken.value == len(arr) * 2 * 128 * 2 ** 23
     assert bad.value == 0
 
 
