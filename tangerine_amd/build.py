@@ -35,10 +35,16 @@ HIP_COMMON = [
     "-Wno-unused-result",
 ]
 
+# The interpreter dispatches on wave-uniform record heads.  Left to the
+# control-flow structurizer, those uniform switches become chains of flag
+# blocks (and the P-points-per-lane kernels four times the code); AMDGPU runs
+# uniform branches natively, so the interpreter units skip structurizing them.
+UNIFORM_CFG = ["-mllvm", "-structurizecfg-skip-uniform-regions=true"]
+
 # (source, extra flags)
 HIP_UNITS = [
-    ("tg_eval_exact.hip", ["-ffp-contract=off"]),
-    ("tg_eval_fast.hip", ["-ffp-contract=off"]),
+    ("tg_eval_exact.hip", ["-ffp-contract=off", *UNIFORM_CFG]),
+    ("tg_eval_fast.hip", ["-ffp-contract=off", *UNIFORM_CFG]),
     ("tg_mesh.hip", ["-ffp-contract=off"]),
     ("tg_capi.hip", ["-ffp-contract=off"]),
     ("tg_export.hip", ["-ffp-contract=off"]),
