@@ -282,7 +282,7 @@ int tg_tracer_set_domain(tg_tracer* tracer, const float lo[3], float size);
  * first_budget steps, 64 rays to a wave; each later pass the rays still
  * marching (sorted by the cell they are in) at most `budget` more, packed so
  * that about resume_waves waves run them, until max_steps.  first_budget <= 0:
- * one pass.  Results do not depend on these.  Default 64, 192, 65536. */
+ * one pass.  Results do not depend on these.  Default 48, 208, 65536. */
 int tg_tracer_set_passes(tg_tracer* tracer, int first_budget, int budget, uint32_t resume_waves);
 /* Renders width*height rays on the device; hit normals by central differences
  * with step normal_h (<= 0: none).  Results stay on the device until copied. */

@@ -41,10 +41,10 @@ struct Transform8
 // the cell they are in, at most kPassBudget steps, packed so that about
 // kResumeWaves waves run them (at most 64 rays, at least one to a wave): the
 // long marches of a frame no longer share a wave, where each waited on the
-// others' cell programs.  (1920x1080, 256 primitives: 15.0 ms in one pass,
-// 12.7 ms as 64 + 192 steps with the 7k rays left one to a wave.)
-constexpr int kFirstBudget = 64;
-constexpr int kPassBudget = 192;
+// others' cell programs.  (1920x1080, 256 primitives: 10.6 ms in one pass,
+// 9.0 ms as 48 + 208 steps with the rays left one to a wave.)
+constexpr int kFirstBudget = 48;
+constexpr int kPassBudget = 208;
 constexpr uint32_t kResumeWaves = 65536;
 constexpr int kKeyBits = 27;  // trace_key: level above an 18-bit Morton code
 
