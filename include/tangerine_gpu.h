@@ -402,6 +402,12 @@ int tg_mesher_autotune(tg_mesher* mesher, const tg_program* program, const tg_gr
                        tg_tune_result* out);
 int tg_mesher_set_tuning(tg_mesher* mesher, uint32_t points_per_lane, uint32_t waves, uint32_t workgroups_per_cu,
                          uint32_t min_prune_len);
+/* Launch shape of culled exports (tg_mesher_export_culled), kept apart from
+ * the plain export's: their near-surface tiles run deeper programs, which
+ * want more spill room per wave (default 4 points per lane, 4 waves, 6
+ * persistent workgroups per CU).  Every setting writes the same samples. */
+int tg_mesher_set_culled_tuning(tg_mesher* mesher, uint32_t points_per_lane, uint32_t waves,
+                                uint32_t workgroups_per_cu);
 
 /* ---- batched export (no reference counterpart; closest: one Sodapop
  * MeshingJob per model on Scheduler workers, sodapop.cpp:214-224,

@@ -53,6 +53,10 @@ struct tg_mesher
 	int points_per_lane = 4;      // block kernel: points per lane (2, 4, 8; tg_mesher_autotune)
 	int waves = 2;                // block kernel: waves per workgroup (1, 2, 4)
 	int workgroups_per_cu = 8;    // 0: a workgroup per block (LDS tables); else persistent, global tables
+	// launch shape of culled exports (tg_mesher_set_culled_tuning): their
+	// near-surface tiles run deeper programs, which want more spill room per
+	// wave than the plain export's tuning may leave
+	int culled_points_per_lane = 4, culled_waves = 4, culled_workgroups_per_cu = 6;
 	int cus = 0;                  // compute units of the device
 	DevBuf persist_args, persist_head, persist_tables;  // the persistent launch's argument copy, queue head, tables
 	DevBuf order_keys, order_values, order, order_tmp;   // its queue: blocks by program cost
@@ -378,14 +382,19 @@ int eval_blocks(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fl
 	unsigned long long* work = m->counters.as<unsigned long long>();
 	TG_CHECK_HIP(hipMemsetAsync(m->counters.ptr, 0, 64, m->stream), "counter reset");
 	BlockEvalArgs ba = block_args(m, p, g, tile_mask, block_list, n_listed, work);
-	int pts = m->points_per_lane;
-	if (m->workgroups_per_cu > 0)
+	// culled exports (a block list) have their own launch shape
+	const bool culled = block_list != nullptr;
+	int pts = culled ? m->culled_points_per_lane : m->points_per_lane;
+	const int shape_waves = culled ? m->culled_waves : m->waves;
+	const int shape_wgs = culled ? m->culled_workgroups_per_cu : m->workgroups_per_cu;
+	ba.waves = shape_waves;
+	if (shape_wgs > 0)
 	{
 		// Persistent launch: workgroups take blocks from a queue; each keeps its
 		// block's decision tables and tile programs in its own slice of global
 		// scratch, LDS holds only tile descriptors and the spill region.
-		const size_t per_wg = kMaxLds / size_t(m->workgroups_per_cu);
-		const int waves = m->waves;
+		const size_t per_wg = kMaxLds / size_t(shape_wgs);
+		const int waves = shape_waves;
 		while (pts > 2 && block_lds_bytes_global(block_spill_slots(ba.max_depth, pts), pts, waves) > per_wg) pts >>= 1;
 		if (block_lds_bytes_global(block_spill_slots(ba.max_depth, pts), pts, waves) > per_wg)
 			return fail(TG_ERR_LIMIT, "block evaluation: spill region too large for the workgroups per CU");
@@ -400,7 +409,7 @@ int eval_blocks(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fl
 		q.max_len = ba.max_len;
 		q.max_depth = ba.max_depth;
 		q.table_words = block_global_words(ba.max_binary, ba.max_len);
-		const uint32_t wgs = uint32_t(std::max(1, m->cus * m->workgroups_per_cu));
+		const uint32_t wgs = uint32_t(std::max(1, m->cus * shape_wgs));
 		rc = m->persist_args.ensure(sizeof(BlockEvalArgs));
 		if (!rc) rc = m->persist_head.ensure(256);
 		if (!rc) rc = m->persist_tables.ensure(size_t(wgs) * q.table_words * 4);
@@ -822,6 +831,20 @@ extern "C" int tg_mesher_set_tuning(tg_mesher* m, uint32_t points_per_lane, uint
 	m->waves = int(waves);
 	m->workgroups_per_cu = int(workgroups_per_cu);
 	m->min_prune_len = min_prune_len;
+	return TG_OK;
+}
+
+extern "C" int tg_mesher_set_culled_tuning(tg_mesher* m, uint32_t points_per_lane, uint32_t waves,
+	uint32_t workgroups_per_cu)
+{
+	if (!m || !(points_per_lane == 2 || points_per_lane == 4 || points_per_lane == 8) ||
+		!(waves == 1 || waves == 2 || waves == 4) || workgroups_per_cu > 16)
+		return fail(TG_ERR_INVALID,
+			"tg_mesher_set_culled_tuning: points per lane in {2,4,8}, waves in {1,2,4}, workgroups per CU 0..16");
+	if (int rc = refuse_batch_owned(m, "tg_mesher_set_culled_tuning")) return rc;
+	m->culled_points_per_lane = int(points_per_lane);
+	m->culled_waves = int(waves);
+	m->culled_workgroups_per_cu = int(workgroups_per_cu);
 	return TG_OK;
 }
 

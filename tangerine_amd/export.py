@@ -77,6 +77,11 @@ class Mesher:
         check(self.lib.tg_mesher_set_tuning(self.h, int(points_per_lane), int(waves), int(workgroups_per_cu),
                                             int(min_prune_len)), "tg_mesher_set_tuning")
 
+    def set_culled_tuning(self, points_per_lane: int, waves: int, workgroups_per_cu: int) -> None:
+        """Launch shape of culled exports (kept apart from the plain export's)."""
+        check(self.lib.tg_mesher_set_culled_tuning(self.h, int(points_per_lane), int(waves), int(workgroups_per_cu)),
+              "tg_mesher_set_culled_tuning")
+
     def _bricks(self, fn, name) -> np.ndarray:
         n = C.c_uint64()
         check(fn(self.h, None, 0, C.byref(n)), name)

@@ -162,6 +162,12 @@ def test_certificate_only_export_is_identical(gpu, fast):
     assert v0.tobytes() == v1.tobytes() and t0.tobytes() == t1.tobytes()
     assert culled["tiles_evaluated"] < 0.5 * plain["tiles_evaluated"]
     assert culled["bricks_certified"] == culled["bricks_flagged"] == kept.sum()
+    # the culled launch shape (its own tuning) changes no byte of the mesh
+    for shape in [(8, 2, 8), (2, 1, 0), (4, 4, 6), (8, 4, 2)]:
+        m.set_culled_tuning(*shape)
+        m.export_culled(prog, None, origin, step, cells, fast=fast)
+        v2, t2 = m.mesh()
+        assert v0.tobytes() == v2.tobytes() and t0.tobytes() == t2.tobytes(), shape
     m.close()
 
 
