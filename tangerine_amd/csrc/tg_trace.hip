@@ -303,7 +303,10 @@ extern "C" int tg_tracer_render(tg_tracer* tr, const tg_program* p, const tg_cam
 			}
 		}
 		// far cube (rays leaving the scene march on through few, small cell
-		// programs), the eye's cube, then four levels over the scene
+		// programs), the eye's cube, then 16^3 and 64^3 cells over the scene.
+		// (1080p, 256 primitives, pruning + marching: 8/16/32/64 levels over
+		// the scene 11.2 ms, 16/64 10.7, 32/64 10.6, 64 alone 10.5, 48/96 12.3:
+		// the waterfall over per-ray cells makes the middle levels cheap to drop.)
 		CubeDomain doms[kTraceLevels];
 		uint32_t ns[kTraceLevels];
 		int count = 0;
@@ -314,7 +317,7 @@ extern "C" int tg_tracer_render(tg_tracer* tr, const tg_program* p, const tg_cam
 		}
 		doms[count] = outer;
 		ns[count++] = 8;
-		for (uint32_t n = 8; n <= 64; n *= 2)
+		for (uint32_t n : { 16u, 64u })
 		{
 			doms[count] = inner;
 			ns[count++] = n;
