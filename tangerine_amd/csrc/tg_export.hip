@@ -162,13 +162,13 @@ int surface_nets_vertices(tg_mesher* m)
 	m->tail[2] = m->tail[3] = 0;
 	if (!nv) return TG_OK;
 	TG_CHECK_HIP(mesh_list_cubes(a, m->stream), "cube list launch");
-	TG_CHECK_HIP(mesh_vertices(a, nv, m->stream), "vertex launch");  // also each cube's quad count
+	TG_CHECK_HIP(mesh_vertices(a, nv, m->stream), "vertex launch");  // also each cube's quad flags
 	size_t tmp = 0;
-	TG_CHECK_HIP(exclusive_scan_u32(nullptr, nullptr, nv, nullptr, tmp, m->stream), "scan sizing");
+	TG_CHECK_HIP(exclusive_scan_quads(nullptr, nullptr, nv, nullptr, tmp, m->stream), "scan sizing");
 	rc = m->scan_tmp.ensure(tmp);
 	if (rc) return rc;
 	tmp = m->scan_tmp.bytes;
-	TG_CHECK_HIP(exclusive_scan_u32(a.cube_quads, a.cube_qbase, nv, m->scan_tmp.ptr, tmp, m->stream), "quad scan");
+	TG_CHECK_HIP(exclusive_scan_quads(a.cube_quads, a.cube_qbase, nv, m->scan_tmp.ptr, tmp, m->stream), "quad scan");
 	TG_CHECK_HIP(hipMemcpyAsync(&m->tail[2], a.cube_qbase + nv - 1, 4, hipMemcpyDeviceToHost, m->stream), "count copy");
 	TG_CHECK_HIP(hipMemcpyAsync(&m->tail[3], a.cube_quads + nv - 1, 4, hipMemcpyDeviceToHost, m->stream), "count copy");
 	return TG_OK;
@@ -177,7 +177,7 @@ int surface_nets_vertices(tg_mesher* m)
 int surface_nets_quads(tg_mesher* m)
 {
 	MeshArgs& a = m->args;
-	const uint64_t quads = uint64_t(m->tail[2]) + m->tail[3];
+	const uint64_t quads = uint64_t(m->tail[2]) + uint64_t(__builtin_popcount(m->tail[3] & 7u));  // + the last cube's
 	m->ntris = quads * 2;
 	int rc = m->triangles.ensure(std::max<uint64_t>(quads, 1) * 24);
 	if (rc) return rc;

@@ -27,8 +27,8 @@ struct MeshArgs
 	uint32_t* word_count;
 	uint32_t* word_base;
 	uint32_t* cubes;              // active cubes in vertex-id order: i + sx (j + sy k)
-	uint32_t* cube_quads;         // quads per active cube, then their exclusive scan
-	uint32_t* cube_qbase;
+	uint32_t* cube_quads;         // quad flags per active cube (tg_mesh.hip: quad_flags)
+	uint32_t* cube_qbase;         // exclusive scan of the quad counts
 	// outputs
 	float* vertices;
 	uint32_t* triangles;
@@ -36,13 +36,16 @@ struct MeshArgs
 
 hipError_t mesh_classify(const MeshArgs& a, hipStream_t s);
 // Thread per active cube (after the vertex-id scan): cube list, vertices,
-// quad counts and, after their scan, triangles.
+// quad flags and, after the scan of their counts, triangles.
 hipError_t mesh_list_cubes(const MeshArgs& a, hipStream_t s);
 hipError_t mesh_vertices(const MeshArgs& a, uint64_t nverts, hipStream_t s);
 hipError_t mesh_quad_write(const MeshArgs& a, uint64_t nverts, hipStream_t s);
 hipError_t mesh_pool_to_dense(const MeshArgs& a, float* dense, uint32_t k0, uint32_t nk, hipStream_t s);
 hipError_t mesh_dense_to_pool(const MeshArgs& a, const float* dense, hipStream_t s);
 hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, void* temp, size_t& temp_bytes, hipStream_t s);
+// Exclusive scan of the quad counts (bits 0..2) of n quad-flag words.
+hipError_t exclusive_scan_quads(const uint32_t* flags, uint32_t* out, uint64_t n, void* temp, size_t& temp_bytes,
+	hipStream_t s);
 // Blocks (all n, or the n of `list`) ordered by their pruned program's cost,
 // most expensive first, into `order`; keys needs 2n words, values n.  With
 // temp = null only sizes temp_bytes.

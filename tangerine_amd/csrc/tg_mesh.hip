@@ -88,13 +88,23 @@ __device__ __forceinline__ void corner_masks(const MeshArgs& a, uint32_t tix, ui
 	any = s | s_up | sz | sz_up;
 }
 
+// Workgroups are dealt to the 8 XCDs in turn (workgroup b to XCD b % 8).  This
+// numbering gives each XCD a contiguous range of logical workgroups, so that
+// the sign words that neighbouring workgroups share are read through one L2.
+__device__ __forceinline__ uint32_t xcd_contiguous(uint32_t b, uint32_t n)
+{
+	const uint32_t per = n / 8, rem = n % 8, x = b % 8, k = b / 8;
+	return x < rem ? x * (per + 1) + k : rem * (per + 1) + (x - rem) * per + k;
+}
+
 __global__ void __launch_bounds__(256) k_classify(MeshArgs a, uint32_t nseg)
 {
 	constexpr uint32_t kTiles = kSegWords * 8 + 1;  // + the next tile, for the x + 1 corner
 	__shared__ uint64_t lds[8 * 8 * kSegWords];      // [kk][jj][8 tiles per word] bytes
 	__shared__ uint64_t all_s[kTiles * 8], any_s[kTiles * 8];
-	const uint32_t seg = blockIdx.x % nseg;
-	const uint32_t cy = (blockIdx.x / nseg) % a.cby, cz = blockIdx.x / (nseg * a.cby);
+	const uint32_t blk = xcd_contiguous(blockIdx.x, gridDim.x);
+	const uint32_t seg = blk % nseg;
+	const uint32_t cy = (blk / nseg) % a.cby, cz = blk / (nseg * a.cby);
 	const uint32_t t0 = seg * kSegWords * 8;
 	for (uint32_t q = threadIdx.x; q < kTiles * 8; q += blockDim.x)
 		corner_masks(a, t0 + (q >> 3), cy, cz, q & 7, all_s[q], any_s[q]);
@@ -146,13 +156,9 @@ __global__ void __launch_bounds__(256) k_classify(MeshArgs a, uint32_t nseg)
 
 // Vertex of active cube (i, j, k): the mean of the edge crossings, mapped to
 // world space.
-__device__ __forceinline__ void place_vertex(const MeshArgs& a, uint32_t i, uint32_t j, uint32_t k, float* v)
+// Corner samples of cube (i, j, k) in the reference cube numbering.
+__device__ __forceinline__ void cube_corners(const MeshArgs& a, uint32_t i, uint32_t j, uint32_t k, float (&s)[8])
 {
-	const float maxx = a.gx + float(a.sx) * a.dx;
-	const float maxy = a.gy + float(a.sy) * a.dy;
-	const float maxz = a.gz + float(a.sz) * a.dz;
-	const float fsx = float(a.sx), fsy = float(a.sy), fsz = float(a.sz);
-	float s[8];
 	s[0] = pool_at(a, i, j, k);
 	s[1] = pool_at(a, i + 1, j, k);
 	s[2] = pool_at(a, i + 1, j + 1, k);
@@ -161,6 +167,15 @@ __device__ __forceinline__ void place_vertex(const MeshArgs& a, uint32_t i, uint
 	s[5] = pool_at(a, i + 1, j, k + 1);
 	s[6] = pool_at(a, i + 1, j + 1, k + 1);
 	s[7] = pool_at(a, i, j + 1, k + 1);
+}
+
+__device__ __forceinline__ void place_vertex(const MeshArgs& a, uint32_t i, uint32_t j, uint32_t k, const float (&s)[8],
+	float* v)
+{
+	const float maxx = a.gx + float(a.sx) * a.dx;
+	const float maxy = a.gy + float(a.sy) * a.dy;
+	const float maxz = a.gz + float(a.sz) * a.dz;
+	const float fsx = float(a.sx), fsy = float(a.sy), fsz = float(a.sz);
 	// corner offsets (x, y, z) and the 12 edges of the reference cube numbering
 	constexpr uint8_t cx[8] = { 0, 1, 1, 0, 0, 1, 1, 0 };
 	constexpr uint8_t cy[8] = { 0, 0, 1, 1, 0, 0, 1, 1 };
@@ -190,24 +205,41 @@ __device__ __forceinline__ void place_vertex(const MeshArgs& a, uint32_t i, uint
 	v[2] = a.gz + (maxz - a.gz) * (gcz - 0.0f) / (fsz - 0.0f);
 }
 
-// Thread per cube-row word: the word's active cubes into the cube list at
-// their vertex ids.
+// Thread per four cube-row words (one 32-byte load when whole): each word's
+// active cubes into the cube list at their vertex ids.
+constexpr uint32_t kListWords = 4;
+
 __global__ void __launch_bounds__(256) k_list_cubes(MeshArgs a)
 {
 	const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-	const uint64_t rows = uint64_t(a.sy) * a.sz;
-	if (t >= rows * a.aw) return;
-	uint64_t act = a.active[t];
-	if (!act) return;
-	const uint32_t w = uint32_t(t % a.aw);
-	const uint64_t row = t / a.aw;
-	const uint32_t base_index = w * 64 + uint32_t(row) * a.sx;  // i + sx (j + sy k) for i = 64 w
-	uint32_t id = a.word_base[t];
-	while (act)
+	const uint64_t total = uint64_t(a.sy) * a.sz * a.aw;
+	const uint64_t w0 = t * kListWords;
+	if (w0 >= total) return;
+	uint64_t act[kListWords] = { 0, 0, 0, 0 };
+	if (w0 + kListWords <= total)
 	{
-		const int b = __ffsll((long long)act) - 1;
-		act &= act - 1;
-		a.cubes[id++] = base_index + uint32_t(b);
+		const ulonglong2* src = reinterpret_cast<const ulonglong2*>(a.active + w0);
+		const ulonglong2 lo = src[0], hi = src[1];
+		act[0] = lo.x; act[1] = lo.y; act[2] = hi.x; act[3] = hi.y;
+	}
+	else
+		for (uint32_t q = 0; q < kListWords && w0 + q < total; ++q) act[q] = a.active[w0 + q];
+#pragma unroll
+	for (uint32_t q = 0; q < kListWords; ++q)
+	{
+		uint64_t bits = act[q];
+		if (!bits) continue;
+		const uint64_t wi = w0 + q;
+		const uint32_t w = uint32_t(wi % a.aw);
+		const uint64_t row = wi / a.aw;
+		const uint32_t base_index = w * 64 + uint32_t(row) * a.sx;  // i + sx (j + sy k) for i = 64 w
+		uint32_t id = a.word_base[wi];
+		while (bits)
+		{
+			const int b = __ffsll((long long)bits) - 1;
+			bits &= bits - 1;
+			a.cubes[id++] = base_index + uint32_t(b);
+		}
 	}
 }
 
@@ -219,69 +251,87 @@ __device__ __forceinline__ void cube_coords(const MeshArgs& a, uint32_t c, uint3
 	k = r / a.sy;
 }
 
-// Quads of one active cube (vertex `self`: active cubes are listed in vertex
-// order): writes up to 6 triangles when `out` is set and returns the quad
-// count.
-__device__ __forceinline__ uint32_t cube_quads(const MeshArgs& a, uint32_t i, uint32_t j, uint32_t k, uint32_t* out,
-	uint32_t self)
+// The three quads of a cube, one per directed edge (0,4), (3,0), (0,1), over
+// its neighbour cubes 0..5 (surface_nets.cpp:415-423): cube c is
+// (i - (c == 0 || c == 1 || c == 5), j - (1 <= c && c <= 3), k - (c >= 3)).
+constexpr uint8_t kQuadNbr[3][3] = { { 0, 1, 2 }, { 0, 5, 4 }, { 2, 3, 4 } };
+
+// Quads of one active cube as flags: bit q (q < 3) when quad q exists (its
+// three neighbour cubes are active), bit 3 + q when the quad's edge has its
+// second value greater than its first (vertex order {0,1,2} rather than
+// {2,1,0}).  s: the cube's corner samples.
+__device__ __forceinline__ uint32_t quad_flags(const MeshArgs& a, uint32_t i, uint32_t j, uint32_t k, const float (&s)[8])
 {
 	if (i == 0 || j == 0 || k == 0) return 0;
-	// neighbour cubes 0..5 (surface_nets.cpp:415-423)
 	const uint32_t nx[6] = { i - 1, i - 1, i, i, i, i - 1 };
 	const uint32_t ny[6] = { j, j - 1, j - 1, j - 1, j, j };
 	const uint32_t nz[6] = { k, k, k, k - 1, k - 1, k - 1 };
-	constexpr uint8_t quad[3][3] = { { 0, 1, 2 }, { 0, 5, 4 }, { 2, 3, 4 } };
-	uint32_t count = 0;
+	// edge (first, second) values: (0,4) = samples (i,j,k), (i,j,k+1);
+	// (3,0) = (i,j+1,k), (i,j,k); (0,1) = (i,j,k), (i+1,j,k)
+	const bool fwd[3] = { s[4] > s[0], s[0] > s[3], s[1] > s[0] };
+	uint32_t flags = 0;
+#pragma unroll
 	for (int q = 0; q < 3; ++q)
 	{
 		bool all = true;
 #pragma unroll
-		for (int t = 0; t < 3; ++t) all = all && cube_active(a, nx[quad[q][t]], ny[quad[q][t]], nz[quad[q][t]]);
-		if (!all) continue;
-		if (out)
-		{
-			float first, second;
-			if (q == 0) { first = pool_at(a, i, j, k); second = pool_at(a, i, j, k + 1); }
-			else if (q == 1) { first = pool_at(a, i, j + 1, k); second = pool_at(a, i, j, k); }
-			else { first = pool_at(a, i, j, k); second = pool_at(a, i + 1, j, k); }
-			uint32_t n[3];
-#pragma unroll
-			for (int t = 0; t < 3; ++t) n[t] = vertex_id(a, nx[quad[q][t]], ny[quad[q][t]], nz[quad[q][t]]);
-			const bool fwd = second > first;
-			const uint32_t v0 = self;
-			const uint32_t v1 = fwd ? n[0] : n[2], v2 = n[1], v3 = fwd ? n[2] : n[0];
-			// two triangles (v0, v1, v2), (v0, v2, v3): 24 bytes at an 8-byte
-			// aligned place (quad bases are whole quads), as three 8-byte stores
-			uint2* tri = reinterpret_cast<uint2*>(out + size_t(count) * 6);
-			tri[0] = make_uint2(v0, v1);
-			tri[1] = make_uint2(v2, v0);
-			tri[2] = make_uint2(v2, v3);
-		}
-		++count;
+		for (int t = 0; t < 3; ++t) all = all && cube_active(a, nx[kQuadNbr[q][t]], ny[kQuadNbr[q][t]], nz[kQuadNbr[q][t]]);
+		if (all) flags |= (1u << q) | (fwd[q] ? 8u << q : 0u);
 	}
-	return count;
+	return flags;
 }
 
-// Thread per active cube: vertex placement and the number of quads the cube
-// emits (scanned before k_quad_write).
+// Thread per active cube: vertex placement and the cube's quad flags (their
+// quad counts are scanned before k_quad_write).
 __global__ void __launch_bounds__(256) k_vertices(MeshArgs a, uint64_t nverts)
 {
 	const uint64_t id = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (id >= nverts) return;
 	uint32_t i, j, k;
 	cube_coords(a, a.cubes[id], i, j, k);
-	place_vertex(a, i, j, k, a.vertices + size_t(id) * 3);
-	a.cube_quads[id] = cube_quads(a, i, j, k, nullptr, uint32_t(id));
+	float s[8];
+	cube_corners(a, i, j, k, s);
+	place_vertex(a, i, j, k, s, a.vertices + size_t(id) * 3);
+	a.cube_quads[id] = quad_flags(a, i, j, k, s);
 }
 
-// Thread per active cube: its quads at the scanned base.
+// Thread per active cube (vertex `id`: active cubes are listed in vertex
+// order): its quads at the scanned base, each as the two triangles
+// (v0, v1, v2), (v0, v2, v3).
 __global__ void __launch_bounds__(256) k_quad_write(MeshArgs a, uint64_t nverts)
 {
 	const uint64_t id = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (id >= nverts) return;
+	const uint32_t flags = a.cube_quads[id];
+	if ((flags & 7u) == 0) return;
 	uint32_t i, j, k;
 	cube_coords(a, a.cubes[id], i, j, k);
-	cube_quads(a, i, j, k, a.triangles + size_t(a.cube_qbase[id]) * 6, uint32_t(id));
+	uint32_t* out = a.triangles + size_t(a.cube_qbase[id]) * 6;
+#pragma unroll
+	for (int q = 0; q < 3; ++q)
+	{
+		if (!((flags >> q) & 1u)) continue;
+		uint32_t n[3];
+#pragma unroll
+		for (int t = 0; t < 3; ++t)
+		{
+			const int c = kQuadNbr[q][t];
+			const uint32_t dx = (c == 0 || c == 1 || c == 5) ? 1u : 0u;
+			const uint32_t dy = (1 <= c && c <= 3) ? 1u : 0u;
+			const uint32_t dz = c >= 3 ? 1u : 0u;
+			n[t] = vertex_id(a, i - dx, j - dy, k - dz);
+		}
+		const bool fwd = (flags >> (3 + q)) & 1u;
+		const uint32_t v0 = uint32_t(id);
+		const uint32_t v1 = fwd ? n[0] : n[2], v2 = n[1], v3 = fwd ? n[2] : n[0];
+		// 24 bytes at an 8-byte aligned place (quad bases are whole quads), as
+		// three 8-byte stores
+		uint2* tri = reinterpret_cast<uint2*>(out);
+		tri[0] = make_uint2(v0, v1);
+		tri[1] = make_uint2(v2, v0);
+		tri[2] = make_uint2(v2, v3);
+		out += 6;
+	}
 }
 
 // Pool tiles -> dense x-fastest grid (for copying samples back to the host).
@@ -346,7 +396,7 @@ hipError_t mesh_classify(const MeshArgs& a, hipStream_t s)
 hipError_t mesh_list_cubes(const MeshArgs& a, hipStream_t s)
 {
 	const uint64_t n = uint64_t(a.sy) * a.sz * a.aw;
-	hipLaunchKernelGGL(k_list_cubes, dim3(blocks_for(n, 256)), dim3(256), 0, s, a);
+	hipLaunchKernelGGL(k_list_cubes, dim3(blocks_for((n + kListWords - 1) / kListWords, 256)), dim3(256), 0, s, a);
 	return hipGetLastError();
 }
 
@@ -383,6 +433,20 @@ hipError_t mesh_dense_to_pool(const MeshArgs& a, const float* dense, hipStream_t
 hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, void* temp, size_t& temp_bytes, hipStream_t s)
 {
 	return hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, in, out, int(n), s);
+}
+
+namespace {
+struct QuadCount
+{
+	__host__ __device__ uint32_t operator()(uint32_t flags) const { return uint32_t(__builtin_popcount(flags & 7u)); }
+};
+}  // namespace
+
+hipError_t exclusive_scan_quads(const uint32_t* flags, uint32_t* out, uint64_t n, void* temp, size_t& temp_bytes,
+	hipStream_t s)
+{
+	hipcub::TransformInputIterator<uint32_t, QuadCount, const uint32_t*> counts(flags, QuadCount{});
+	return hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, counts, out, int(n), s);
 }
 
 namespace {
