@@ -64,6 +64,7 @@ struct CellPrograms
 	uint32_t max_len = 0;     // most entries of a packed cell program
 	uint64_t whole_cells = 0; // cells left with the whole program (pool overflow)
 	bool valid = false;       // false: the program is not prunable here
+	bool reran = false;       // the last pruning emitted again after a pool overflow (desc changed)
 	PruneState state;
 	unsigned long long* host = nullptr;  // pinned copy of the pass counters
 	void release()

@@ -47,7 +47,11 @@ struct tg_mesher
 	CellPrograms blocks;
 	MeshArgs args{};
 	uint64_t nverts = 0, ntris = 0;
-	uint32_t* tail = nullptr;     // pinned: vertex and quad totals read back by surface nets
+	// room for the next mesh, from the last one (0: none yet): with it the
+	// surface-net stages run without waiting for their counts
+	uint64_t spec_verts = 0, spec_quads = 0;
+	uint32_t* tail = nullptr;     // pinned: vertex and quad totals read back by surface nets, then
+	                              // (from word 4) the block evaluation's two work counters
 	bool have_grid = false;
 	tg_export_stats stats{};
 	int points_per_lane = 4;      // block kernel: points per lane (2, 4, 8; tg_mesher_autotune)
@@ -159,6 +163,7 @@ int surface_nets_vertices(tg_mesher* m)
 	a.cubes = m->cubes.as<uint32_t>();
 	a.cube_quads = m->quad_count.as<uint32_t>();
 	a.cube_qbase = m->quad_base.as<uint32_t>();
+	a.cap_verts = nv;
 	m->tail[2] = m->tail[3] = 0;
 	if (!nv) return TG_OK;
 	TG_CHECK_HIP(mesh_list_cubes(a, m->stream), "cube list launch");
@@ -174,27 +179,104 @@ int surface_nets_vertices(tg_mesher* m)
 	return TG_OK;
 }
 
+// Quads of the vertex stage's flags (m->tail[2..3]: the last cube's scanned
+// base and its flags).
+uint64_t quads_from_tail(const tg_mesher* m)
+{
+	return uint64_t(m->tail[2]) + uint64_t(__builtin_popcount(m->tail[3] & 7u));
+}
+
 int surface_nets_quads(tg_mesher* m)
 {
 	MeshArgs& a = m->args;
-	const uint64_t quads = uint64_t(m->tail[2]) + uint64_t(__builtin_popcount(m->tail[3] & 7u));  // + the last cube's
+	const uint64_t quads = quads_from_tail(m);
 	m->ntris = quads * 2;
 	int rc = m->triangles.ensure(std::max<uint64_t>(quads, 1) * 24);
 	if (rc) return rc;
 	a.triangles = m->triangles.as<uint32_t>();
+	a.cap_quads = quads;
 	if (quads) TG_CHECK_HIP(mesh_quad_write(a, m->nverts, m->stream), "quad write launch");
 	return TG_OK;
 }
 
+// All three stages at once, in the room m->spec_verts / m->spec_quads that an
+// earlier mesh left: the kernels take the vertex count from the device and
+// write within the room, and the host reads the counts once, at the end.
+// `fits` is false when the mesh outgrew the room (it is then incomplete).
+int surface_nets_speculative(tg_mesher* m, bool& fits)
+{
+	MeshArgs& a = m->args;
+	fits = false;
+	const uint64_t cap_v = m->spec_verts, cap_q = m->spec_quads;
+	// every buffer sized before the first launch (growing one frees it)
+	size_t tmp = 0;
+	TG_CHECK_HIP(exclusive_scan_quads(nullptr, nullptr, cap_v, nullptr, tmp, m->stream), "scan sizing");
+	int rc = m->scan_tmp.ensure(tmp);
+	if (!rc) rc = m->vertices.ensure(cap_v * 12);
+	if (!rc) rc = m->cubes.ensure(cap_v * 4);
+	if (!rc) rc = m->quad_count.ensure(cap_v * 4);
+	if (!rc) rc = m->quad_base.ensure(cap_v * 4);
+	if (!rc) rc = m->triangles.ensure(cap_q * 24);
+	if (rc) return rc;
+	rc = surface_nets_classify(m);
+	if (rc) return rc;
+	a.vertices = m->vertices.as<float>();
+	a.cubes = m->cubes.as<uint32_t>();
+	a.cube_quads = m->quad_count.as<uint32_t>();
+	a.cube_qbase = m->quad_base.as<uint32_t>();
+	a.triangles = m->triangles.as<uint32_t>();
+	a.cap_verts = cap_v;
+	a.cap_quads = cap_q;
+	TG_CHECK_HIP(mesh_list_cubes(a, m->stream), "cube list launch");
+	TG_CHECK_HIP(mesh_vertices(a, cap_v, m->stream), "vertex launch");
+	tmp = m->scan_tmp.bytes;
+	TG_CHECK_HIP(exclusive_scan_quads(a.cube_quads, a.cube_qbase, cap_v, m->scan_tmp.ptr, tmp, m->stream), "quad scan");
+	TG_CHECK_HIP(hipMemcpyAsync(&m->tail[2], a.cube_qbase + cap_v - 1, 4, hipMemcpyDeviceToHost, m->stream), "count copy");
+	TG_CHECK_HIP(hipMemcpyAsync(&m->tail[3], a.cube_quads + cap_v - 1, 4, hipMemcpyDeviceToHost, m->stream), "count copy");
+	TG_CHECK_HIP(mesh_quad_write(a, cap_v, m->stream), "quad write launch");
+	TG_CHECK_HIP(hipStreamSynchronize(m->stream), "surface nets");
+	const uint64_t nv = uint64_t(m->tail[0]) + m->tail[1];
+	if (nv > cap_v) return TG_OK;
+	// flags past the active cubes are zero, so the scan's total is the mesh's
+	const uint64_t quads = quads_from_tail(m);
+	if (quads > cap_q) return TG_OK;
+	m->nverts = nv;
+	m->ntris = quads * 2;
+	fits = true;
+	return TG_OK;
+}
+
+// Room for the next mesh of this mesher: a quarter more than this one.
+void size_next_mesh(tg_mesher* m)
+{
+	m->spec_verts = m->nverts + m->nverts / 4 + 4096;
+	m->spec_quads = m->ntris / 2 + m->ntris / 8 + 4096;
+}
+
 int surface_nets(tg_mesher* m)
 {
+	if (m->spec_verts > 0)
+	{
+		bool fits = false;
+		int rc = surface_nets_speculative(m, fits);
+		if (rc) return rc;
+		if (fits)
+		{
+			// a much smaller mesh than the room: shrink it for the next one
+			if (m->nverts * 2 < m->spec_verts) size_next_mesh(m);
+			return TG_OK;
+		}
+	}
 	int rc = surface_nets_classify(m);
 	if (rc) return rc;
 	TG_CHECK_HIP(hipStreamSynchronize(m->stream), "vertex count");
 	rc = surface_nets_vertices(m);
 	if (rc) return rc;
 	TG_CHECK_HIP(hipStreamSynchronize(m->stream), "quad count");
-	return surface_nets_quads(m);
+	rc = surface_nets_quads(m);
+	if (rc) return rc;
+	size_next_mesh(m);
+	return TG_OK;
 }
 
 // Dynamic LDS a workgroup may ask for: the CU's 160 KiB less room for the
@@ -372,10 +454,34 @@ BlockEvalArgs block_args(tg_mesher* m, const tg_program* p, const tg_grid* g, co
 	return ba;
 }
 
+// The block evaluation's work counters, copied back into the pinned words 4..7
+// of m->tail.
+unsigned long long* work_readback(tg_mesher* m) { return reinterpret_cast<unsigned long long*>(m->tail + 4); }
+
+// The persistent launch's queue: the n blocks (all, or those of `list`) by
+// their program's cost, most expensive first, into m->order.
+int block_order(tg_mesher* m, const BlockDesc* desc, const uint32_t* list, uint32_t n)
+{
+	size_t tmp = 0;
+	TG_CHECK_HIP(blocks_by_cost(desc, list, n, nullptr, nullptr, nullptr, nullptr, tmp, m->stream), "sort sizing");
+	int rc = m->order_keys.ensure(std::max<size_t>(n, 1) * 8);
+	if (!rc) rc = m->order_values.ensure(std::max<size_t>(n, 1) * 4);
+	if (!rc) rc = m->order.ensure(std::max<size_t>(n, 1) * 4);
+	if (!rc) rc = m->order_tmp.ensure(std::max<size_t>(tmp, 1));
+	if (rc) return rc;
+	tmp = m->order_tmp.bytes;
+	TG_CHECK_HIP(blocks_by_cost(desc, list, n, m->order_keys.as<uint32_t>(), m->order_values.as<uint32_t>(),
+		m->order.as<uint32_t>(), m->order_tmp.ptr, tmp, m->stream), "block order");
+	return TG_OK;
+}
+
 // Pruned evaluation (k_eval_blocks) of the listed blocks' masked tiles, or of
-// every tile.  Needs m->blocks.valid.
+// every tile.  Needs m->blocks.valid.  `sorted`: m->order already holds the
+// queue of every block (block_order, enqueued with the pruning).
+// Asynchronous: once the caller has synchronised the stream, eval_blocks_read
+// gives the work counters and the launch's duration.
 int eval_blocks(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t flags, const uint64_t* tile_mask,
-	const uint32_t* block_list, uint32_t n_listed, uint64_t* evals, double* flops)
+	const uint32_t* block_list, uint32_t n_listed, bool sorted = false)
 {
 	int rc = m->counters.ensure(64);
 	if (rc) return rc;
@@ -419,21 +525,12 @@ int eval_blocks(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fl
 		q.tables = m->persist_tables.as<uint32_t>();
 		// the queue takes the most expensive block programs first, so that its
 		// tail holds short ones
+		if (!sorted)
 		{
-			const uint32_t nb = ba.nblocks;
-			size_t tmp = 0;
-			TG_CHECK_HIP(blocks_by_cost(ba.desc, ba.block_list, nb, nullptr, nullptr, nullptr, nullptr, tmp, m->stream),
-				"sort sizing");
-			rc = m->order_keys.ensure(std::max<size_t>(nb, 1) * 8);
-			if (!rc) rc = m->order_values.ensure(std::max<size_t>(nb, 1) * 4);
-			if (!rc) rc = m->order.ensure(std::max<size_t>(nb, 1) * 4);
-			if (!rc) rc = m->order_tmp.ensure(std::max<size_t>(tmp, 1));
+			rc = block_order(m, ba.desc, ba.block_list, ba.nblocks);
 			if (rc) return rc;
-			tmp = m->order_tmp.bytes;
-			TG_CHECK_HIP(blocks_by_cost(ba.desc, ba.block_list, nb, m->order_keys.as<uint32_t>(),
-				m->order_values.as<uint32_t>(), m->order.as<uint32_t>(), m->order_tmp.ptr, tmp, m->stream), "block order");
-			ba.block_list = m->order.as<const uint32_t>();
 		}
+		ba.block_list = m->order.as<const uint32_t>();
 		TG_CHECK_HIP(hipMemcpyAsync(m->persist_args.ptr, &ba, sizeof(ba), hipMemcpyHostToDevice, m->stream), "upload");
 		m->last_points_per_lane = pts;
 		m->last_waves = waves;
@@ -450,14 +547,7 @@ int eval_blocks(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fl
 		         : exact::launch_batch(q, pts, uint32_t(waves), wgs, m->stream);
 		if (e != hipSuccess) return hip_fail(e, "block evaluation launch");
 		TG_CHECK_HIP(hipEventRecord(m->ev[5], m->stream), "event");
-		unsigned long long wk[2] = { 0, 0 };
-		TG_CHECK_HIP(hipMemcpyAsync(wk, work, 16, hipMemcpyDeviceToHost, m->stream), "work read");
-		TG_CHECK_HIP(hipStreamSynchronize(m->stream), "block evaluation");
-		float kms = 0;
-		TG_CHECK_HIP(hipEventElapsedTime(&kms, m->ev[4], m->ev[5]), "timing");
-		m->last_block_ms = kms;
-		*evals = wk[0];
-		*flops = double(wk[1]);
+		TG_CHECK_HIP(hipMemcpyAsync(work_readback(m), work, 16, hipMemcpyDeviceToHost, m->stream), "work read");
 		return TG_OK;
 	}
 	// Fit the LDS: fewer points per lane first, then fewer waves.
@@ -484,14 +574,18 @@ int eval_blocks(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fl
 	                                      : exact::launch_eval_blocks(ba, pts, m->stream);
 	if (e != hipSuccess) return hip_fail(e, "block evaluation launch");
 	TG_CHECK_HIP(hipEventRecord(m->ev[5], m->stream), "event");
-	unsigned long long wk[2] = { 0, 0 };
-	TG_CHECK_HIP(hipMemcpyAsync(wk, work, 16, hipMemcpyDeviceToHost, m->stream), "work read");
-	TG_CHECK_HIP(hipStreamSynchronize(m->stream), "block evaluation");
+	TG_CHECK_HIP(hipMemcpyAsync(work_readback(m), work, 16, hipMemcpyDeviceToHost, m->stream), "work read");
+	return TG_OK;
+}
+
+// Results of the last eval_blocks, after the stream has been synchronised.
+int eval_blocks_read(tg_mesher* m, uint64_t* evals, double* flops)
+{
 	float kms = 0;
 	TG_CHECK_HIP(hipEventElapsedTime(&kms, m->ev[4], m->ev[5]), "timing");
 	m->last_block_ms = kms;
-	*evals = wk[0];
-	*flops = double(wk[1]);
+	*evals = work_readback(m)[0];
+	*flops = double(work_readback(m)[1]);
 	return TG_OK;
 }
 
@@ -510,7 +604,7 @@ extern "C" int tg_mesher_create(tg_mesher** out)
 	if (e == hipSuccess) e = hipDeviceGetAttribute(&m->cus, hipDeviceAttributeMultiprocessorCount, dev);
 	if (e == hipSuccess) e = hipStreamCreate(&m->stream);
 	for (int i = 0; e == hipSuccess && i < 6; ++i) e = hipEventCreate(&m->ev[i]);
-	if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&m->tail), 16, hipHostMallocDefault);
+	if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&m->tail), 32, hipHostMallocDefault);
 	if (e != hipSuccess)
 	{
 		m->tail = nullptr;
@@ -694,10 +788,23 @@ int run_export(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fla
 
 	TG_CHECK_HIP(hipEventRecord(m->ev[0], m->stream), "event");
 	m->blocks.valid = false;
+	bool sorted = false;
 	if (!(flags & TG_FLAG_NO_PRUNE) && p->decoded.prunable())
 	{
-		rc = prune_blocks(m, p, g, flags);
+		rc = prune_blocks_start(m, p, g, flags);
 		if (rc) return rc;
+		// The persistent launch's queue of every block, enqueued behind the
+		// pruning so that the host issues it while the pruning runs; a rerun
+		// of the emission changes the programs, and the queue is sorted again.
+		if (!culled && m->workgroups_per_cu > 0 && m->blocks.state.pending)
+		{
+			rc = block_order(m, m->blocks.desc.as<const BlockDesc>(), nullptr, uint32_t(m->blocks.state.ncells));
+			if (rc) return rc;
+			sorted = true;
+		}
+		rc = prune_blocks_finish(m, p, g, flags);
+		if (rc) return rc;
+		sorted = sorted && !m->blocks.reran;
 	}
 	const bool blocks = m->blocks.valid;
 	Selection sel;
@@ -712,8 +819,8 @@ int run_export(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fla
 	if (blocks)
 	{
 		rc = culled ? eval_blocks(m, p, g, flags, m->tile_mask.as<const uint64_t>(), m->block_list.as<const uint32_t>(),
-		                          sel.listed, &evals, &flops)
-		            : eval_blocks(m, p, g, flags, nullptr, nullptr, 0, &evals, &flops);
+		                          sel.listed)
+		            : eval_blocks(m, p, g, flags, nullptr, nullptr, 0, sorted);
 		if (rc) return rc;
 	}
 	else
@@ -735,6 +842,11 @@ int run_export(tg_mesher* m, const tg_program* p, const tg_grid* g, uint32_t fla
 	if (rc) return rc;
 	TG_CHECK_HIP(hipEventRecord(m->ev[2], m->stream), "event");
 	TG_CHECK_HIP(hipEventSynchronize(m->ev[2]), "export");
+	if (blocks)
+	{
+		rc = eval_blocks_read(m, &evals, &flops);
+		if (rc) return rc;
+	}
 	float t_pre = 0, t_eval = 0, t_mesh = 0, t_all = 0;
 	TG_CHECK_HIP(hipEventElapsedTime(&t_pre, m->ev[0], m->ev[3]), "timing");
 	TG_CHECK_HIP(hipEventElapsedTime(&t_eval, m->ev[3], m->ev[1]), "timing");
@@ -878,10 +990,8 @@ extern "C" int tg_mesher_autotune(tg_mesher* m, const tg_program* p, const tg_gr
 			double ms = 0.0;
 			for (int rep = 0; rep < 3; ++rep)
 			{
-				uint64_t evals = 0;
-				double flops = 0.0;
 				TG_CHECK_HIP(hipEventRecord(m->ev[0], m->stream), "event");
-				rc = eval_blocks(m, p, g, flags, nullptr, nullptr, 0, &evals, &flops);
+				rc = eval_blocks(m, p, g, flags, nullptr, nullptr, 0);
 				if (rc) return rc;
 				TG_CHECK_HIP(hipEventRecord(m->ev[1], m->stream), "event");
 				TG_CHECK_HIP(hipEventSynchronize(m->ev[1]), "tuning");
@@ -916,9 +1026,12 @@ extern "C" int tg_mesher_autotune(tg_mesher* m, const tg_program* p, const tg_gr
 		double ms = 0.0;
 		for (int rep = 0; rep < 3; ++rep)
 		{
+			rc = eval_blocks(m, p, g, flags, nullptr, nullptr, 0);
+			if (rc) return rc;
+			TG_CHECK_HIP(hipStreamSynchronize(m->stream), "tuning");
 			uint64_t evals = 0;
 			double flops = 0.0;
-			rc = eval_blocks(m, p, g, flags, nullptr, nullptr, 0, &evals, &flops);
+			rc = eval_blocks_read(m, &evals, &flops);
 			if (rc) return rc;
 			if (rep > 0) ms += m->last_block_ms;
 		}
@@ -958,7 +1071,7 @@ extern "C" int tg_mesher_mesh_values(tg_mesher* m, const float* values, const tg
 	TG_CHECK_HIP(hipEventRecord(m->ev[2], m->stream), "event");
 	TG_CHECK_HIP(hipEventSynchronize(m->ev[2]), "mesh");
 	float t_mesh = 0;
-	hipEventElapsedTime(&t_mesh, m->ev[1], m->ev[2]);
+	TG_CHECK_HIP(hipEventElapsedTime(&t_mesh, m->ev[1], m->ev[2]), "timing");
 	m->stats = tg_export_stats{};
 	m->stats.mesh_ms = t_mesh;
 	m->stats.total_ms = t_mesh;

@@ -29,6 +29,8 @@ struct MeshArgs
 	uint32_t* cubes;              // active cubes in vertex-id order: i + sx (j + sy k)
 	uint32_t* cube_quads;         // quad flags per active cube (tg_mesh.hip: quad_flags)
 	uint32_t* cube_qbase;         // exclusive scan of the quad counts
+	uint64_t cap_verts;           // room for active cubes (cubes, vertices, quad flags and bases)
+	uint64_t cap_quads;           // room for quads (triangle pairs)
 	// outputs
 	float* vertices;
 	uint32_t* triangles;

@@ -238,7 +238,8 @@ __global__ void __launch_bounds__(256) k_list_cubes(MeshArgs a)
 		{
 			const int b = __ffsll((long long)bits) - 1;
 			bits &= bits - 1;
-			a.cubes[id++] = base_index + uint32_t(b);
+			if (id < a.cap_verts) a.cubes[id] = base_index + uint32_t(b);
+			++id;
 		}
 	}
 }
@@ -281,12 +282,26 @@ __device__ __forceinline__ uint32_t quad_flags(const MeshArgs& a, uint32_t i, ui
 	return flags;
 }
 
-// Thread per active cube: vertex placement and the cube's quad flags (their
-// quad counts are scanned before k_quad_write).
-__global__ void __launch_bounds__(256) k_vertices(MeshArgs a, uint64_t nverts)
+// Active cubes (vertices) of the classified grid: the scanned word counts' total.
+__device__ __forceinline__ uint64_t active_cubes(const MeshArgs& a)
+{
+	const uint64_t last = uint64_t(a.sy) * a.sz * a.aw - 1;
+	return uint64_t(a.word_base[last]) + a.word_count[last];
+}
+
+// Thread per active cube (launched for n >= the active cubes, within the
+// room cap_verts): vertex placement and the cube's quad flags (their quad
+// counts are scanned before k_quad_write); threads past the active cubes
+// write zero flags.
+__global__ void __launch_bounds__(256) k_vertices(MeshArgs a, uint64_t n)
 {
 	const uint64_t id = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-	if (id >= nverts) return;
+	if (id >= n) return;
+	if (id >= active_cubes(a))
+	{
+		a.cube_quads[id] = 0;
+		return;
+	}
 	uint32_t i, j, k;
 	cube_coords(a, a.cubes[id], i, j, k);
 	float s[8];
@@ -296,17 +311,20 @@ __global__ void __launch_bounds__(256) k_vertices(MeshArgs a, uint64_t nverts)
 }
 
 // Thread per active cube (vertex `id`: active cubes are listed in vertex
-// order): its quads at the scanned base, each as the two triangles
-// (v0, v1, v2), (v0, v2, v3).
-__global__ void __launch_bounds__(256) k_quad_write(MeshArgs a, uint64_t nverts)
+// order; launched like k_vertices): its quads at the scanned base, each as
+// the two triangles (v0, v1, v2), (v0, v2, v3), when they fit the room
+// cap_quads.
+__global__ void __launch_bounds__(256) k_quad_write(MeshArgs a, uint64_t n)
 {
 	const uint64_t id = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-	if (id >= nverts) return;
+	if (id >= n) return;
 	const uint32_t flags = a.cube_quads[id];
 	if ((flags & 7u) == 0) return;
+	const uint32_t base = a.cube_qbase[id];
+	if (uint64_t(base) + uint32_t(__popc(flags & 7u)) > a.cap_quads) return;  // no room: the caller redoes the mesh
 	uint32_t i, j, k;
 	cube_coords(a, a.cubes[id], i, j, k);
-	uint32_t* out = a.triangles + size_t(a.cube_qbase[id]) * 6;
+	uint32_t* out = a.triangles + size_t(base) * 6;
 #pragma unroll
 	for (int q = 0; q < 3; ++q)
 	{
@@ -453,13 +471,25 @@ namespace {
 
 // Thread per block (or per listed block): its program's cost (keys) and its
 // index (values) for the sort below.
+// Sort key of a block program's cost: its position on a log scale with eight
+// steps per octave (8 bits), so that one radix pass orders the queue; costs
+// within an eighth of an octave keep their block order (the sort is stable).
+__device__ __forceinline__ uint32_t cost_key(uint32_t flops)
+{
+	const uint32_t c = flops + 1u - (flops == 0xffffffffu ? 1u : 0u);
+	const uint32_t e = 31u - uint32_t(__builtin_clz(c));
+	const uint32_t frac = e >= 3u ? (c >> (e - 3u)) & 7u : (c << (3u - e)) & 7u;
+	return e * 8u + frac;
+}
+constexpr int kCostKeyBits = 8;
+
 __global__ void __launch_bounds__(256) k_block_costs(const BlockDesc* desc, const uint32_t* list, uint32_t n,
 	uint32_t* keys, uint32_t* values)
 {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
 	const uint32_t b = list ? list[i] : i;
-	keys[i] = desc[b].flops;
+	keys[i] = cost_key(desc[b].flops);
 	values[i] = b;
 }
 
@@ -484,7 +514,7 @@ __global__ void __launch_bounds__(256) k_batch_keys(const BlockEvalArgs* trees, 
 			hi = mid;
 	}
 	const uint32_t b = i - first[lo];
-	keys[i] = trees[lo].desc[b].flops;
+	keys[i] = cost_key(trees[lo].desc[b].flops);
 	values[i] = (lo << 20) | b;
 }
 
@@ -508,14 +538,14 @@ hipError_t batch_queue_by_cost(const BlockEvalArgs* trees, const uint32_t* first
 	uint32_t* values_out = sort_buf + 3 * size_t(total);
 	if (!temp)
 		return hipcub::DeviceRadixSort::SortPairsDescending(temp, temp_bytes, keys, keys_out, values, values_out,
-			int(total), 0, 32, s);
+			int(total), 0, kCostKeyBits, s);
 	if (total == 0) return hipSuccess;
 	hipLaunchKernelGGL(k_batch_keys, dim3((total + 255) / 256), dim3(256), 0, s, trees, first, ntrees, total, keys,
 		values);
 	hipError_t e = hipGetLastError();
 	if (e != hipSuccess) return e;
 	e = hipcub::DeviceRadixSort::SortPairsDescending(temp, temp_bytes, keys, keys_out, values, values_out, int(total),
-		0, 32, s);
+		0, kCostKeyBits, s);
 	if (e != hipSuccess) return e;
 	hipLaunchKernelGGL(k_batch_items, dim3((total + 255) / 256), dim3(256), 0, s, values_out, total, items);
 	return hipGetLastError();
@@ -533,7 +563,7 @@ hipError_t blocks_by_cost(const BlockDesc* desc, const uint32_t* list, uint32_t 
 		if (e != hipSuccess) return e;
 	}
 	return hipcub::DeviceRadixSort::SortPairsDescending(temp, temp_bytes, keys, keys_out, values, order, int(n), 0,
-		32, s);
+		kCostKeyBits, s);
 }
 
 }  // namespace tg

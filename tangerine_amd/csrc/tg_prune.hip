@@ -374,12 +374,14 @@ int prune_cells_finish(hipStream_t stream, const tg_program* p, const float orig
 	PruneState& ps = out.state;
 	if (!ps.pending) return TG_OK;
 	ps.pending = false;
+	out.reran = false;
 	hipError_t e = hipStreamSynchronize(stream);
 	if (e != hipSuccess) return hip_fail(e, "cell pruning");
 	const unsigned long long* c = out.host;
 	if (c[0] > ps.capacity && c[0] * sizeof(Entry) <= (size_t(8) << 30))
 	{
 		// some cells did not fit: rerun the emission with the exact total
+		out.reran = true;
 		ps.capacity = c[0];
 		int rc = prune_pass(stream, p, origin, cell, counts, fast, out, false);
 		if (rc) return rc;
