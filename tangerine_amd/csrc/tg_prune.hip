@@ -115,7 +115,15 @@ LevelArgs level_args(const PruneArgs& pa, const CellPrograms& out)
 	return la;
 }
 
-// Height step `step` (1 .. 2 heights - 1) of the level-parallel pass: values
+// Steps of the level-parallel pass: heights 1 .. H - 1 upward, then H - 1 ..
+// 1 downward (the parents count their leaf children), or, for a leaf root
+// (H = 1), the root's own liveness step.
+size_t level_steps(const PruneState& ps)
+{
+	return ps.heights >= 2 ? 2 * ps.heights - 1 : 2 * ps.heights;
+}
+
+// Height step `step` (1 .. level_steps - 1) of the level-parallel pass: values
 // and decisions upward from height 1 (leaf operands are evaluated in place),
 // then liveness and lengths downward.
 void level_step(LevelArgs& la, const CellPrograms& out, size_t step)
@@ -150,7 +158,7 @@ int prune_pass(hipStream_t stream, const tg_program* p, const float origin[3], c
 		LevelArgs la = level_args(pa, out);
 		e = hipMemsetAsync(out.lens.ptr, 0, waves * 64 * 4, stream);
 		if (e != hipSuccess) return hip_fail(e, "length reset");
-		for (size_t step = 1; step < 2 * ps.heights; ++step)
+		for (size_t step = 1; step < level_steps(ps); ++step)
 		{
 			level_step(la, out, step);
 			e = fast ? fast::launch_prune_level(la, stream) : exact::launch_prune_level(la, stream);
@@ -271,7 +279,7 @@ int prune_cells_many(hipStream_t stream, const tg_program* const* progs, const C
 		if (out.state.level_parallel)
 		{
 			TG_CHECK_HIP(hipMemsetAsync(out.lens.ptr, 0, out.state.waves * 64 * 4, stream), "length reset");
-			steps = std::max(steps, 2 * out.state.heights);
+			steps = std::max(steps, level_steps(out.state));
 		}
 		pas[i] = prune_args(progs[i], grids[i].origin, grids[i].cell, grids[i].counts, out);
 	}
@@ -299,7 +307,7 @@ int prune_cells_many(hipStream_t stream, const tg_program* const* progs, const C
 		for (uint32_t i = 0; i < n; ++i)
 		{
 			const CellPrograms& out = *outs[i];
-			if (!ready[i] || !out.state.level_parallel || step >= 2 * out.state.heights) continue;
+			if (!ready[i] || !out.state.level_parallel || step >= level_steps(out.state)) continue;
 			LevelArgs la = level_args(pas[i], out);
 			level_step(la, out, step);
 			const uint32_t per_entry = (la.ngroups + la.groups_per_wave - 1) / la.groups_per_wave;
