@@ -148,16 +148,22 @@ int surface_nets_classify(tg_mesher* m)
 	return TG_OK;
 }
 
+// Room for a mesh after one of n vertices (or quads): a quarter more.
+uint64_t room_for(uint64_t n) { return n + n / 4 + 4096; }
+
 int surface_nets_vertices(tg_mesher* m)
 {
 	MeshArgs& a = m->args;
 	m->nverts = uint64_t(m->tail[0]) + m->tail[1];
 	const uint64_t nv = m->nverts;
 	if (nv >= (1ull << 31)) return fail(TG_ERR_LIMIT, "surface nets: more than 2^31 vertices");
-	int rc = m->vertices.ensure(std::max<uint64_t>(nv, 1) * 12);
-	if (!rc) rc = m->cubes.ensure(std::max<uint64_t>(nv, 1) * 4);
-	if (!rc) rc = m->quad_count.ensure(std::max<uint64_t>(nv, 1) * 4);
-	if (!rc) rc = m->quad_base.ensure(std::max<uint64_t>(nv, 1) * 4);
+	// sized for the room the next mesh will run in (size_next_mesh), so
+	// that it allocates nothing
+	const uint64_t room = room_for(nv);
+	int rc = m->vertices.ensure(room * 12);
+	if (!rc) rc = m->cubes.ensure(room * 4);
+	if (!rc) rc = m->quad_count.ensure(room * 4);
+	if (!rc) rc = m->quad_base.ensure(room * 4);
 	if (rc) return rc;
 	a.vertices = m->vertices.as<float>();
 	a.cubes = m->cubes.as<uint32_t>();
@@ -169,7 +175,7 @@ int surface_nets_vertices(tg_mesher* m)
 	TG_CHECK_HIP(mesh_list_cubes(a, m->stream), "cube list launch");
 	TG_CHECK_HIP(mesh_vertices(a, nv, m->stream), "vertex launch");  // also each cube's quad flags
 	size_t tmp = 0;
-	TG_CHECK_HIP(exclusive_scan_quads(nullptr, nullptr, nv, nullptr, tmp, m->stream), "scan sizing");
+	TG_CHECK_HIP(exclusive_scan_quads(nullptr, nullptr, room, nullptr, tmp, m->stream), "scan sizing");
 	rc = m->scan_tmp.ensure(tmp);
 	if (rc) return rc;
 	tmp = m->scan_tmp.bytes;
@@ -191,7 +197,7 @@ int surface_nets_quads(tg_mesher* m)
 	MeshArgs& a = m->args;
 	const uint64_t quads = quads_from_tail(m);
 	m->ntris = quads * 2;
-	int rc = m->triangles.ensure(std::max<uint64_t>(quads, 1) * 24);
+	int rc = m->triangles.ensure(room_for(quads) * 24);
 	if (rc) return rc;
 	a.triangles = m->triangles.as<uint32_t>();
 	a.cap_quads = quads;
@@ -249,8 +255,8 @@ int surface_nets_speculative(tg_mesher* m, bool& fits)
 // Room for the next mesh of this mesher: a quarter more than this one.
 void size_next_mesh(tg_mesher* m)
 {
-	m->spec_verts = m->nverts + m->nverts / 4 + 4096;
-	m->spec_quads = m->ntris / 2 + m->ntris / 8 + 4096;
+	m->spec_verts = room_for(m->nverts);
+	m->spec_quads = room_for(m->ntris / 2);
 }
 
 int surface_nets(tg_mesher* m)
