@@ -107,6 +107,9 @@ int layout(tg_mesher* m, const tg_grid* g)
 	a.tx = (a.nx + 7) / 8; a.ty = (a.ny + 7) / 8; a.tz = (a.nz + 7) / 8;
 	a.cbx = (a.sx + 7) / 8; a.cby = (a.sy + 7) / 8; a.cbz = (a.sz + 7) / 8;
 	a.aw = (a.sx + 63) / 64;
+	a.sx_div = mesh_div_magic(a.sx);
+	a.sy_div = mesh_div_magic(a.sy);
+	a.aw_div = mesh_div_magic(a.aw);
 	const uint64_t tiles = uint64_t(a.tx) * a.ty * a.tz;
 	int rc = m->pool.ensure(tiles * kTileSamples * sizeof(float));
 	if (!rc) rc = m->signs.ensure(tiles * 64);

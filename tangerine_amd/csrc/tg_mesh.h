@@ -15,6 +15,7 @@ struct MeshArgs
 	uint32_t sx, sy, sz;          // cubes per axis
 	uint32_t nx, ny, nz;          // samples per axis (= s + 1)
 	uint32_t tx, ty, tz;          // sample tiles per axis
+	uint64_t sx_div, sy_div, aw_div;  // division magics of sx, sy, aw (mesh_div_magic)
 	uint32_t cbx, cby, cbz;       // cube bricks per axis (8^3 cubes each)
 	// inputs
 	const float* pool;
@@ -35,6 +36,11 @@ struct MeshArgs
 	float* vertices;
 	uint32_t* triangles;
 };
+
+// n / d for every 32-bit n is floor(M n / 2^64) with M = ceil(2^64 / d) (Lemire,
+// Kaser and Kurz, "Faster remainder by direct computation", Software:
+// Practice and Experience 49(6), 2019); M = 0 stands for d = 1.
+inline uint64_t mesh_div_magic(uint32_t d) { return d <= 1 ? 0ull : ~0ull / d + 1ull; }
 
 hipError_t mesh_classify(const MeshArgs& a, hipStream_t s);
 // Thread per active cube (after the vertex-id scan): cube list, vertices,

@@ -46,6 +46,15 @@ __device__ __forceinline__ float pool_at(const MeshArgs& a, uint32_t i, uint32_t
 	return a.pool[size_t(slot) * kTileSamples + ((k & 7) * 64 + (j & 7) * 8 + (i & 7))];
 }
 
+// n / d from d's magic (mesh_div_magic: M = ceil(2^64 / d), 0 for d = 1).
+__device__ __forceinline__ uint32_t div_magic(uint32_t n, uint64_t m)
+{
+	if (m == 0) return n;
+	const uint64_t lo = uint64_t(uint32_t(m)) * n;
+	const uint64_t hi = (m >> 32) * n + (lo >> 32);
+	return uint32_t(hi >> 32);
+}
+
 __device__ __forceinline__ bool cube_active(const MeshArgs& a, uint32_t i, uint32_t j, uint32_t k)
 {
 	const uint64_t word = a.active[(size_t(k) * a.sy + j) * a.aw + (i >> 6)];
@@ -185,18 +194,22 @@ __device__ __forceinline__ void place_vertex(const MeshArgs& a, uint32_t i, uint
 	const float fi = float(i), fj = float(j), fk = float(k);
 	float sx_ = 0.0f, sy_ = 0.0f, sz_ = 0.0f;
 	int n = 0;
+	// Every edge is computed and a non-crossing one adds +0.0 (no branches:
+	// a wave's cubes cross most edges between them).  The sums start at +0.0
+	// and every term is >= 0 (coordinates of the cube's corners, t in
+	// [0, 1]), so adding +0.0 leaves them bit for bit as skipping would.
 #pragma unroll
 	for (int e = 0; e < 12; ++e)
 	{
 		const float v1 = s[e0[e]], v2 = s[e1[e]];
-		if ((v1 >= 0.0f) == (v2 >= 0.0f)) continue;
+		const bool cross = (v1 >= 0.0f) != (v2 >= 0.0f);
 		const float p1x = fi + float(cx[e0[e]]), p1y = fj + float(cy[e0[e]]), p1z = fk + float(cz[e0[e]]);
 		const float p2x = fi + float(cx[e1[e]]), p2y = fj + float(cy[e1[e]]), p2z = fk + float(cz[e1[e]]);
 		const float tt = (0.0f - v1) / (v2 - v1);
-		sx_ = sx_ + (p1x + tt * (p2x - p1x));
-		sy_ = sy_ + (p1y + tt * (p2y - p1y));
-		sz_ = sz_ + (p1z + tt * (p2z - p1z));
-		++n;
+		sx_ = sx_ + (cross ? p1x + tt * (p2x - p1x) : 0.0f);
+		sy_ = sy_ + (cross ? p1y + tt * (p2y - p1y) : 0.0f);
+		sz_ = sz_ + (cross ? p1z + tt * (p2z - p1z) : 0.0f);
+		n += cross ? 1 : 0;
 	}
 	const float fn = float(n);
 	const float gcx = sx_ / fn, gcy = sy_ / fn, gcz = sz_ / fn;
@@ -229,10 +242,10 @@ __global__ void __launch_bounds__(256) k_list_cubes(MeshArgs a)
 	{
 		uint64_t bits = act[q];
 		if (!bits) continue;
-		const uint64_t wi = w0 + q;
-		const uint32_t w = uint32_t(wi % a.aw);
-		const uint64_t row = wi / a.aw;
-		const uint32_t base_index = w * 64 + uint32_t(row) * a.sx;  // i + sx (j + sy k) for i = 64 w
+		const uint32_t wi = uint32_t(w0 + q);  // words < 2^31
+		const uint32_t row = div_magic(wi, a.aw_div);
+		const uint32_t w = wi - row * a.aw;
+		const uint32_t base_index = w * 64 + row * a.sx;  // i + sx (j + sy k) for i = 64 w
 		uint32_t id = a.word_base[wi];
 		while (bits)
 		{
@@ -246,10 +259,10 @@ __global__ void __launch_bounds__(256) k_list_cubes(MeshArgs a)
 
 __device__ __forceinline__ void cube_coords(const MeshArgs& a, uint32_t c, uint32_t& i, uint32_t& j, uint32_t& k)
 {
-	i = c % a.sx;
-	const uint32_t r = c / a.sx;
-	j = r % a.sy;
-	k = r / a.sy;
+	const uint32_t r = div_magic(c, a.sx_div);
+	i = c - r * a.sx;
+	k = div_magic(r, a.sy_div);
+	j = r - k * a.sy;
 }
 
 // The three quads of a cube, one per directed edge (0,4), (3,0), (0,1), over
