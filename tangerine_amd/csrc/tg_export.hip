@@ -188,6 +188,13 @@ int surface_nets_vertices(tg_mesher* m)
 	return TG_OK;
 }
 
+// Room for the next mesh of this mesher: a quarter more than this one.
+void size_next_mesh(tg_mesher* m)
+{
+	m->spec_verts = room_for(m->nverts);
+	m->spec_quads = room_for(m->ntris / 2);
+}
+
 // Quads of the vertex stage's flags (m->tail[2..3]: the last cube's scanned
 // base and its flags).
 uint64_t quads_from_tail(const tg_mesher* m)
@@ -210,12 +217,11 @@ int surface_nets_quads(tg_mesher* m)
 
 // All three stages at once, in the room m->spec_verts / m->spec_quads that an
 // earlier mesh left: the kernels take the vertex count from the device and
-// write within the room, and the host reads the counts once, at the end.
-// `fits` is false when the mesh outgrew the room (it is then incomplete).
-int surface_nets_speculative(tg_mesher* m, bool& fits)
+// write within the room, and the host reads the counts once, at the end
+// (surface_nets_room_check, after the stream is synchronised).
+int surface_nets_in_room(tg_mesher* m)
 {
 	MeshArgs& a = m->args;
-	fits = false;
 	const uint64_t cap_v = m->spec_verts, cap_q = m->spec_quads;
 	// every buffer sized before the first launch (growing one frees it)
 	size_t tmp = 0;
@@ -243,39 +249,29 @@ int surface_nets_speculative(tg_mesher* m, bool& fits)
 	TG_CHECK_HIP(hipMemcpyAsync(&m->tail[2], a.cube_qbase + cap_v - 1, 4, hipMemcpyDeviceToHost, m->stream), "count copy");
 	TG_CHECK_HIP(hipMemcpyAsync(&m->tail[3], a.cube_quads + cap_v - 1, 4, hipMemcpyDeviceToHost, m->stream), "count copy");
 	TG_CHECK_HIP(mesh_quad_write(a, cap_v, m->stream), "quad write launch");
-	TG_CHECK_HIP(hipStreamSynchronize(m->stream), "surface nets");
-	const uint64_t nv = uint64_t(m->tail[0]) + m->tail[1];
-	if (nv > cap_v) return TG_OK;
-	// flags past the active cubes are zero, so the scan's total is the mesh's
-	const uint64_t quads = quads_from_tail(m);
-	if (quads > cap_q) return TG_OK;
-	m->nverts = nv;
-	m->ntris = quads * 2;
-	fits = true;
 	return TG_OK;
 }
 
-// Room for the next mesh of this mesher: a quarter more than this one.
-void size_next_mesh(tg_mesher* m)
+// After surface_nets_in_room and a wait for the stream: whether the mesh fit
+// the room (else it is incomplete and must be made the waiting way); sets the
+// counts when it did.
+bool surface_nets_room_check(tg_mesher* m)
 {
-	m->spec_verts = room_for(m->nverts);
-	m->spec_quads = room_for(m->ntris / 2);
+	const uint64_t nv = uint64_t(m->tail[0]) + m->tail[1];
+	if (nv > m->spec_verts) return false;
+	// flags past the active cubes are zero, so the scan's total is the mesh's
+	const uint64_t quads = quads_from_tail(m);
+	if (quads > m->spec_quads) return false;
+	m->nverts = nv;
+	m->ntris = quads * 2;
+	// a much smaller mesh than the room: shrink it for the next one
+	if (m->nverts * 2 < m->spec_verts) size_next_mesh(m);
+	return true;
 }
 
-int surface_nets(tg_mesher* m)
+// The waiting way: each stage after the previous stage's counts.
+int surface_nets_waiting(tg_mesher* m)
 {
-	if (m->spec_verts > 0)
-	{
-		bool fits = false;
-		int rc = surface_nets_speculative(m, fits);
-		if (rc) return rc;
-		if (fits)
-		{
-			// a much smaller mesh than the room: shrink it for the next one
-			if (m->nverts * 2 < m->spec_verts) size_next_mesh(m);
-			return TG_OK;
-		}
-	}
 	int rc = surface_nets_classify(m);
 	if (rc) return rc;
 	TG_CHECK_HIP(hipStreamSynchronize(m->stream), "vertex count");
@@ -286,6 +282,19 @@ int surface_nets(tg_mesher* m)
 	if (rc) return rc;
 	size_next_mesh(m);
 	return TG_OK;
+}
+
+
+int surface_nets(tg_mesher* m)
+{
+	if (m->spec_verts > 0)
+	{
+		int rc = surface_nets_in_room(m);
+		if (rc) return rc;
+		TG_CHECK_HIP(hipStreamSynchronize(m->stream), "surface nets");
+		if (surface_nets_room_check(m)) return TG_OK;
+	}
+	return surface_nets_waiting(m);
 }
 
 // Dynamic LDS a workgroup may ask for: the CU's 160 KiB less room for the
@@ -2010,23 +2019,44 @@ extern "C" int tg_batch_export(tg_batch* b, const tg_program* const* programs, c
 	double mesh_ms = 0;
 	uint64_t evals = 0;
 	double flops = 0;
-	// surface nets: each stage over every tree (on the trees' streams) before waiting
+	// surface nets on the trees' streams: in the room an earlier export left
+	// (all stages of every such tree issued before any wait), and the others,
+	// with those that outgrew their room, the waiting way, each stage over
+	// every tree before waiting
+	std::vector<uint8_t> done(count, 0);
+	for (uint32_t i = 0; i < count; ++i)
+		if (b->meshers[i]->spec_verts > 0)
+		{
+			rc = surface_nets_in_room(b->meshers[i]);
+			if (rc) return rc;
+			done[i] = 1;
+		}
+	for (uint32_t i = 0; i < count; ++i)
+		if (done[i])
+		{
+			TG_CHECK_HIP(hipStreamSynchronize(b->meshers[i]->stream), "surface nets");
+			done[i] = surface_nets_room_check(b->meshers[i]) ? 1 : 0;
+		}
 	for (uint32_t i = 0; i < count; ++i)
 	{
+		if (done[i]) continue;
 		rc = surface_nets_classify(b->meshers[i]);
 		if (rc) return rc;
 	}
 	for (uint32_t i = 0; i < count; ++i)
 	{
+		if (done[i]) continue;
 		TG_CHECK_HIP(hipStreamSynchronize(b->meshers[i]->stream), "vertex count");
 		rc = surface_nets_vertices(b->meshers[i]);
 		if (rc) return rc;
 	}
 	for (uint32_t i = 0; i < count; ++i)
 	{
+		if (done[i]) continue;
 		TG_CHECK_HIP(hipStreamSynchronize(b->meshers[i]->stream), "quad count");
 		rc = surface_nets_quads(b->meshers[i]);
 		if (rc) return rc;
+		size_next_mesh(b->meshers[i]);
 	}
 	for (uint32_t i = 0; i < count; ++i)
 	{

@@ -59,6 +59,31 @@ def test_batch_matches_single_exports(gpu, fast):
     b.close()
 
 
+def test_batch_exports_in_the_room_of_the_last(gpu):
+    """A batch's later exports make each tree's mesh in the room its last
+    mesh left: the same trees again (every mesh fits), then the trees in
+    reverse order (meshes that outgrow their tree slot's room are made again
+    the waiting way); every mesh equals its single-tree export byte for byte."""
+    from tangerine_amd.batch import Batch
+    from tangerine_amd.export import Mesher
+    progs, grids = _case()
+    single = Mesher()
+    expect = []
+    for p, g in zip(progs, grids):
+        single.export(p, *g)
+        expect.append(single.mesh())
+    single.close()
+    b = Batch()
+    for order in [list(range(len(progs))), list(range(len(progs))), list(reversed(range(len(progs)))),
+                  list(range(len(progs)))]:
+        b.export([progs[i] for i in order], [grids[i] for i in order])
+        for slot, i in enumerate(order):
+            v, t = expect[i]
+            v1, t1 = b.mesher(slot).mesh()
+            assert v1.tobytes() == v.tobytes() and t1.tobytes() == t.tobytes(), (order, slot)
+    b.close()
+
+
 def test_batch_tunings_agree(gpu):
     from tangerine_amd.batch import Batch
     progs, grids = _case()
