@@ -52,6 +52,7 @@ struct tg_mesher
 	uint64_t spec_verts = 0, spec_quads = 0;
 	uint32_t* tail = nullptr;     // pinned: vertex and quad totals read back by surface nets, then
 	                              // (from word 4) the block evaluation's two work counters
+	uint32_t* tail_dev = nullptr; // the same words as the kernels address them
 	bool have_grid = false;
 	tg_export_stats stats{};
 	int points_per_lane = 4;      // block kernel: points per lane (2, 4, 8; tg_mesher_autotune)
@@ -128,7 +129,7 @@ int layout(tg_mesher* m, const tg_grid* g)
 // first two ends with an asynchronous copy of two counts into the pinned
 // m->tail, which the next stage reads after the caller has synchronised the
 // stream (the batch runs each stage over all its trees before waiting).
-int surface_nets_classify(tg_mesher* m)
+int surface_nets_classify(tg_mesher* m, bool copy_counts = true)
 {
 	MeshArgs& a = m->args;
 	const uint64_t words = uint64_t(a.sy) * a.sz * a.aw;
@@ -146,8 +147,13 @@ int surface_nets_classify(tg_mesher* m)
 	TG_CHECK_HIP(mesh_classify(a, m->stream), "classify launch");
 	tmp = m->scan_tmp.bytes;
 	TG_CHECK_HIP(exclusive_scan_u32(a.word_count, a.word_base, words, m->scan_tmp.ptr, tmp, m->stream), "vertex scan");
-	TG_CHECK_HIP(hipMemcpyAsync(&m->tail[0], a.word_base + words - 1, 4, hipMemcpyDeviceToHost, m->stream), "count copy");
-	TG_CHECK_HIP(hipMemcpyAsync(&m->tail[1], a.word_count + words - 1, 4, hipMemcpyDeviceToHost, m->stream), "count copy");
+	if (copy_counts)
+	{
+		TG_CHECK_HIP(hipMemcpyAsync(&m->tail[0], a.word_base + words - 1, 4, hipMemcpyDeviceToHost, m->stream),
+			"count copy");
+		TG_CHECK_HIP(hipMemcpyAsync(&m->tail[1], a.word_count + words - 1, 4, hipMemcpyDeviceToHost, m->stream),
+			"count copy");
+	}
 	return TG_OK;
 }
 
@@ -233,7 +239,7 @@ int surface_nets_in_room(tg_mesher* m)
 	if (!rc) rc = m->quad_base.ensure(cap_v * 4);
 	if (!rc) rc = m->triangles.ensure(cap_q * 24);
 	if (rc) return rc;
-	rc = surface_nets_classify(m);
+	rc = surface_nets_classify(m, false);
 	if (rc) return rc;
 	a.vertices = m->vertices.as<float>();
 	a.cubes = m->cubes.as<uint32_t>();
@@ -242,13 +248,13 @@ int surface_nets_in_room(tg_mesher* m)
 	a.triangles = m->triangles.as<uint32_t>();
 	a.cap_verts = cap_v;
 	a.cap_quads = cap_q;
+	a.tail = m->tail_dev;  // the kernels write the totals: no copies to issue
 	TG_CHECK_HIP(mesh_list_cubes(a, m->stream), "cube list launch");
 	TG_CHECK_HIP(mesh_vertices(a, cap_v, m->stream), "vertex launch");
 	tmp = m->scan_tmp.bytes;
 	TG_CHECK_HIP(exclusive_scan_quads(a.cube_quads, a.cube_qbase, cap_v, m->scan_tmp.ptr, tmp, m->stream), "quad scan");
-	TG_CHECK_HIP(hipMemcpyAsync(&m->tail[2], a.cube_qbase + cap_v - 1, 4, hipMemcpyDeviceToHost, m->stream), "count copy");
-	TG_CHECK_HIP(hipMemcpyAsync(&m->tail[3], a.cube_quads + cap_v - 1, 4, hipMemcpyDeviceToHost, m->stream), "count copy");
 	TG_CHECK_HIP(mesh_quad_write(a, cap_v, m->stream), "quad write launch");
+	a.tail = nullptr;
 	return TG_OK;
 }
 
@@ -623,6 +629,7 @@ extern "C" int tg_mesher_create(tg_mesher** out)
 	if (e == hipSuccess) e = hipStreamCreate(&m->stream);
 	for (int i = 0; e == hipSuccess && i < 6; ++i) e = hipEventCreate(&m->ev[i]);
 	if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&m->tail), 32, hipHostMallocDefault);
+	if (e == hipSuccess) e = hipHostGetDevicePointer(reinterpret_cast<void**>(&m->tail_dev), m->tail, 0);
 	if (e != hipSuccess)
 	{
 		m->tail = nullptr;

@@ -32,6 +32,9 @@ struct MeshArgs
 	uint32_t* cube_qbase;         // exclusive scan of the quad counts
 	uint64_t cap_verts;           // room for active cubes (cubes, vertices, quad flags and bases)
 	uint64_t cap_quads;           // room for quads (triangle pairs)
+	uint32_t* tail;               // non-null: pinned host words the kernels write the totals to
+	                              // (k_vertices: the last word's vertex base and count,
+	                              // k_quad_write: the last cube's quad base and flags)
 	// outputs
 	float* vertices;
 	uint32_t* triangles;

@@ -310,6 +310,12 @@ __global__ void __launch_bounds__(256) k_vertices(MeshArgs a, uint64_t n)
 {
 	const uint64_t id = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (id >= n) return;
+	if (id == 0 && a.tail)
+	{
+		const uint64_t last = uint64_t(a.sy) * a.sz * a.aw - 1;
+		a.tail[0] = a.word_base[last];
+		a.tail[1] = a.word_count[last];
+	}
 	if (id >= active_cubes(a))
 	{
 		a.cube_quads[id] = 0;
@@ -331,6 +337,11 @@ __global__ void __launch_bounds__(256) k_quad_write(MeshArgs a, uint64_t n)
 {
 	const uint64_t id = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (id >= n) return;
+	if (id == 0 && a.tail)
+	{
+		a.tail[2] = a.cube_qbase[n - 1];
+		a.tail[3] = a.cube_quads[n - 1];
+	}
 	const uint32_t flags = a.cube_quads[id];
 	if ((flags & 7u) == 0) return;
 	const uint32_t base = a.cube_qbase[id];
