@@ -408,6 +408,13 @@ int tg_mesher_set_tuning(tg_mesher* mesher, uint32_t points_per_lane, uint32_t w
  * persistent workgroups per CU).  Every setting writes the same samples. */
 int tg_mesher_set_culled_tuning(tg_mesher* mesher, uint32_t points_per_lane, uint32_t waves,
                                 uint32_t workgroups_per_cu);
+/* How the mesher's block pruning emits each block's program once its
+ * decision and liveness passes ran: 0 automatic (the default: a wave per
+ * block, with a lane per program entry, on grids of at most 2048 blocks;
+ * otherwise the walk), 1 one wave walking the entries of 16 blocks at a time,
+ * 2 a wave per block.  Every setting emits the same programs, so exports give
+ * the same samples and mesh. */
+int tg_mesher_set_emission(tg_mesher* mesher, uint32_t mode);
 
 /* ---- batched export (no reference counterpart; closest: one Sodapop
  * MeshingJob per model on Scheduler workers, sodapop.cpp:214-224,

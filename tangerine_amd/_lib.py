@@ -149,6 +149,7 @@ SIGNATURES = {
     "tg_mesher_autotune": (C.c_int, [_p, _p, C.POINTER(TgGrid), _u32, C.POINTER(TgTuneResult)]),
     "tg_mesher_set_tuning": (C.c_int, [_p, _u32, _u32, _u32, _u32]),
     "tg_mesher_set_culled_tuning": (C.c_int, [_p, _u32, _u32, _u32]),
+    "tg_mesher_set_emission": (C.c_int, [_p, _u32]),
     "tg_mesher_write_stl": (C.c_int, [_p, _p, C.c_char_p, _f, _u32]),
     "tg_mesher_write_ply": (C.c_int, [_p, _p, C.c_char_p, _f, _u32]),
     "tg_mesher_grid_get": (C.c_int, [_p, C.POINTER(TgGrid)]),

@@ -65,6 +65,7 @@ struct CellPrograms
 	uint64_t whole_cells = 0; // cells left with the whole program (pool overflow)
 	bool valid = false;       // false: the program is not prunable here
 	bool reran = false;       // the last pruning emitted again after a pool overflow (desc changed)
+	int emission = 0;         // after level-parallel passes: 0 automatic, 1 sequential walk, 2 a wave per cell
 	PruneState state;
 	unsigned long long* host = nullptr;  // pinned copy of the pass counters
 	void release()

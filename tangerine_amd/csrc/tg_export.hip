@@ -971,6 +971,15 @@ extern "C" int tg_mesher_set_tuning(tg_mesher* m, uint32_t points_per_lane, uint
 	return TG_OK;
 }
 
+extern "C" int tg_mesher_set_emission(tg_mesher* m, uint32_t mode)
+{
+	if (!m) return fail(TG_ERR_INVALID, "tg_mesher_set_emission: null mesher");
+	if (mode > 2) return fail(TG_ERR_INVALID, "tg_mesher_set_emission: mode 0 (automatic), 1 (walk) or 2 (per block)");
+	if (int rc = refuse_batch_owned(m, "tg_mesher_set_emission")) return rc;
+	m->blocks.emission = int(mode);
+	return TG_OK;
+}
+
 extern "C" int tg_mesher_set_culled_tuning(tg_mesher* m, uint32_t points_per_lane, uint32_t waves,
 	uint32_t workgroups_per_cu)
 {

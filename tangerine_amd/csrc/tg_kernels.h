@@ -100,6 +100,7 @@ struct PruneArgs
 	int masks_ready;          // decision masks and liveness already in scratch (k_prune_level): skip both passes
 	const uint32_t* counts;   // with masks_ready: program length per lane, [group][64]
 	uint32_t splits;          // with masks_ready: waves per cell group (1, 2 or 4), each emitting 64 / splits columns
+	int columns;              // with masks_ready: a wave per cell, a lane per entry (k_prune_columns)
 };
 
 // Level-parallel decision pass: one launch per entry height (leaves 0, a
@@ -334,6 +335,9 @@ hipError_t launch_prune_level_multi(const LevelArgs* jobs, const uint32_t* wave_
 // many programs' emissions in one launch: job k owns blocks [block_base[k], ...) of `blocks`
 hipError_t launch_prune_cells_multi(const PruneArgs* jobs, const uint32_t* block_base, uint32_t njobs, uint32_t blocks,
 	uint32_t max_slots, hipStream_t s);
+// the same a wave per cell (PruneArgs::columns): job k owns cells [cell_base[k], ...) of `cells`
+hipError_t launch_prune_columns_multi(const PruneArgs* jobs, const uint32_t* cell_base, uint32_t njobs, uint32_t cells,
+	uint32_t max_entries, hipStream_t s);
 hipError_t launch_vox(const VoxArgs& a, int stage, hipStream_t s);  // 0 select, 1 list
 hipError_t launch_pivot(const PivotArgs& a, hipStream_t s);
 hipError_t launch_cloud(const CloudArgs& a, int stage, hipStream_t s);  // 0 select, 1 points, 2 refine
@@ -360,6 +364,9 @@ hipError_t launch_prune_level_multi(const LevelArgs* jobs, const uint32_t* wave_
 // many programs' emissions in one launch: job k owns blocks [block_base[k], ...) of `blocks`
 hipError_t launch_prune_cells_multi(const PruneArgs* jobs, const uint32_t* block_base, uint32_t njobs, uint32_t blocks,
 	uint32_t max_slots, hipStream_t s);
+// the same a wave per cell (PruneArgs::columns): job k owns cells [cell_base[k], ...) of `cells`
+hipError_t launch_prune_columns_multi(const PruneArgs* jobs, const uint32_t* cell_base, uint32_t njobs, uint32_t cells,
+	uint32_t max_entries, hipStream_t s);
 hipError_t launch_vox(const VoxArgs& a, int stage, hipStream_t s);  // 0 select, 1 list
 hipError_t launch_pivot(const PivotArgs& a, hipStream_t s);
 hipError_t launch_cloud(const CloudArgs& a, int stage, hipStream_t s);  // 0 select, 1 points, 2 refine

@@ -57,6 +57,9 @@ bool entry_levels(const DecodedProgram& d, std::vector<uint32_t>& order, std::ve
 
 }  // namespace
 
+// Grids of at most this many cells emit a wave per cell (PruneArgs::columns).
+constexpr uint64_t kColumnCells = 2048;
+
 namespace {
 
 // One emission pass (and, on the first, the level-parallel decision passes)
@@ -91,6 +94,11 @@ PruneArgs prune_args(const tg_program* p, const float origin[3], const float cel
 	// emission only: four waves per cell group while that leaves under ~8
 	// waves per SIMD
 	pa.splits = (ps.level_parallel && ps.waves * 4 <= 8192) ? 4u : 1u;
+	// A wave per cell (a lane per entry) scans the whole program for every
+	// cell: it wins on grids of few cells, whose programs are long (the walk's
+	// time is the longest walk), and loses on many cells with short programs
+	// (1024^3 blocks: 32,768 cells, 2.3 ms against the walk's 1.0 ms).
+	pa.columns = (ps.level_parallel && (out.emission == 2 || (out.emission == 0 && ps.ncells <= kColumnCells))) ? 1 : 0;
 	return pa;
 }
 
@@ -320,25 +328,41 @@ int prune_cells_many(hipStream_t stream, const tg_program* const* progs, const C
 		const size_t b = put(base.data(), base.size() * 4);
 		level_launches.push_back(Launch{ j, b, uint32_t(jobs.size()), waves });
 	}
-	std::vector<PruneArgs> ejobs;
-	std::vector<uint32_t> ebase;
-	uint32_t blocks = 0, max_slots = 1;
+	// emission jobs: the sequential walk's (a wave per group part) and the
+	// column emission's (a wave per cell), one launch each
+	std::vector<PruneArgs> ejobs, cjobs;
+	std::vector<uint32_t> ebase, cbase;
+	uint32_t blocks = 0, max_slots = 1, cells = 0, max_entries = 1;
 	for (uint32_t i = 0; i < n; ++i)
 	{
 		if (!ready[i]) continue;
 		const PruneArgs& pa = pas[i];
 		const uint32_t ncells = pa.nx * pa.ny * pa.nz;
+		if (pa.masks_ready && pa.columns)
+		{
+			cbase.push_back(cells);
+			cells += ncells;
+			max_entries = std::max(max_entries, pa.n_entries);
+			cjobs.push_back(pa);
+			continue;
+		}
 		const uint32_t splits = pa.masks_ready ? pa.splits : 1u;
 		ebase.push_back(blocks);
 		blocks += (ncells + pa.cells_per_wave - 1) / pa.cells_per_wave * splits;
 		max_slots = std::max(max_slots, pa.slots);
 		ejobs.push_back(pa);
 	}
-	size_t ej = 0, eb = 0;
+	cbase.push_back(cells);
+	size_t ej = 0, eb = 0, cj = 0, cb = 0;
 	if (!ejobs.empty())
 	{
 		ej = put(ejobs.data(), ejobs.size() * sizeof(PruneArgs));
 		eb = put(ebase.data(), ebase.size() * 4);
+	}
+	if (!cjobs.empty())
+	{
+		cj = put(cjobs.data(), cjobs.size() * sizeof(PruneArgs));
+		cb = put(cbase.data(), cbase.size() * 4);
 	}
 	if (!blob.empty())
 	{
@@ -363,6 +387,15 @@ int prune_cells_many(hipStream_t stream, const tg_program* const* progs, const C
 		const hipError_t e =
 			fast ? fast::launch_prune_cells_multi(jobs, base, uint32_t(ejobs.size()), blocks, max_slots, stream)
 			     : exact::launch_prune_cells_multi(jobs, base, uint32_t(ejobs.size()), blocks, max_slots, stream);
+		if (e != hipSuccess) return hip_fail(e, "cell pruning launch");
+	}
+	if (!cjobs.empty())
+	{
+		const PruneArgs* jobs = reinterpret_cast<const PruneArgs*>(dev + cj);
+		const uint32_t* base = reinterpret_cast<const uint32_t*>(dev + cb);
+		const hipError_t e =
+			fast ? fast::launch_prune_columns_multi(jobs, base, uint32_t(cjobs.size()), cells, max_entries, stream)
+			     : exact::launch_prune_columns_multi(jobs, base, uint32_t(cjobs.size()), cells, max_entries, stream);
 		if (e != hipSuccess) return hip_fail(e, "cell pruning launch");
 	}
 	for (uint32_t i = 0; i < n; ++i)

@@ -77,6 +77,12 @@ class Mesher:
         check(self.lib.tg_mesher_set_tuning(self.h, int(points_per_lane), int(waves), int(workgroups_per_cu),
                                             int(min_prune_len)), "tg_mesher_set_tuning")
 
+    def set_emission(self, mode: int) -> None:
+        """Block-program emission after the pruning passes: 0 automatic (a
+        wave per block on grids of at most 2048 blocks, else the walk), 1 a
+        wave walking 16 blocks' entries, 2 a wave per block (same programs)."""
+        check(self.lib.tg_mesher_set_emission(self.h, int(mode)), "tg_mesher_set_emission")
+
     def set_culled_tuning(self, points_per_lane: int, waves: int, workgroups_per_cu: int) -> None:
         """Launch shape of culled exports (kept apart from the plain export's)."""
         check(self.lib.tg_mesher_set_culled_tuning(self.h, int(points_per_lane), int(waves), int(workgroups_per_cu)),
