@@ -57,7 +57,8 @@ bool entry_levels(const DecodedProgram& d, std::vector<uint32_t>& order, std::ve
 
 }  // namespace
 
-// Grids of at most this many cells emit a wave per cell (PruneArgs::columns).
+// Grids of at most this many cells (twice as many for programs of 1024
+// entries or more) emit a wave per cell (PruneArgs::columns).
 constexpr uint64_t kColumnCells = 2048;
 
 namespace {
@@ -98,7 +99,8 @@ PruneArgs prune_args(const tg_program* p, const float origin[3], const float cel
 	// cell: it wins on grids of few cells, whose programs are long (the walk's
 	// time is the longest walk), and loses on many cells with short programs
 	// (1024^3 blocks: 32,768 cells, 2.3 ms against the walk's 1.0 ms).
-	pa.columns = (ps.level_parallel && (out.emission == 2 || (out.emission == 0 && ps.ncells <= kColumnCells))) ? 1 : 0;
+	const bool few = ps.ncells <= kColumnCells || (ps.ncells <= 2 * kColumnCells && d.entries.size() >= 1024);
+	pa.columns = (ps.level_parallel && (out.emission == 2 || (out.emission == 0 && few))) ? 1 : 0;
 	return pa;
 }
 
