@@ -410,10 +410,11 @@ int tg_mesher_set_culled_tuning(tg_mesher* mesher, uint32_t points_per_lane, uin
                                 uint32_t workgroups_per_cu);
 /* How the mesher's block pruning emits each block's program once its
  * decision and liveness passes ran: 0 automatic (the default: a wave per
- * block, with a lane per program entry, on grids of at most 2048 blocks (4096 for programs of 1024 entries or more);
- * otherwise the walk), 1 one wave walking the entries of 16 blocks at a time,
- * 2 a wave per block.  Every setting emits the same programs, so exports give
- * the same samples and mesh. */
+ * block, with a lane per program entry, on grids of at most 2048 blocks, or
+ * 4096 for programs of 1024 entries or more; otherwise the walk), 1 one wave
+ * walking the entries of 16 blocks at a time, 2 a wave per block.  Every
+ * setting emits the same programs, so exports give the same samples and
+ * mesh. */
 int tg_mesher_set_emission(tg_mesher* mesher, uint32_t mode);
 
 /* ---- batched export (no reference counterpart; closest: one Sodapop

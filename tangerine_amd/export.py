@@ -79,7 +79,8 @@ class Mesher:
 
     def set_emission(self, mode: int) -> None:
         """Block-program emission after the pruning passes: 0 automatic (a
-        wave per block on grids of at most 2048 blocks, else the walk), 1 a
+        wave per block on grids of at most 2048 blocks, or 4096 for programs
+        of 1024 entries or more, else the walk), 1 a
         wave walking 16 blocks' entries, 2 a wave per block (same programs)."""
         check(self.lib.tg_mesher_set_emission(self.h, int(mode)), "tg_mesher_set_emission")
 
