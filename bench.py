@@ -19,7 +19,7 @@ of the same line (--legs picks them; default all):
   fast        the headline export in fast numerics (fused multiply-adds,
               hardware square root): its FP32 roofline beside the exact one
   surrogate   surrogate fit, then culled exports (surrogate + certificate,
-              certificate alone, surrogate alone) judged against the plain one
+              certificate alone, surrogate alone) compared with the plain one
   batch64     64 seeded trees (8..2048 primitives, {64..384}^3 over [-6,6]^3):
               cold and warm whole-batch exports, and the captured persistent
               evaluation replayed
@@ -520,7 +520,7 @@ def main(argv=None) -> int:
         fms = fres["block_ms"] if prune and fres["block_ms"] > 0 else fres["eval_ms"]
         out["fast"] = {"ms_per_step": fres["ms_per_step"], "samples_per_s": samples / (fres["ms_per_step"] * 1e-3),
                        "autotune": ftune, "roofline": roofline(fst["flops"], fms, block_kernel(fst, prune) + " (fast numerics)")}
-        # back to the exact export (and its launch shape) for the surrogate leg's judge
+        # back to the exact export (and its launch shape) for the surrogate leg's comparison
         if tuning:
             mesher.set_tuning(tuning["points_per_lane"], tuning["waves"], tuning.get("min_prune_len", 0),
                               tuning.get("workgroups_per_cu", 0))
