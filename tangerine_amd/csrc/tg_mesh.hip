@@ -115,11 +115,15 @@ __global__ void __launch_bounds__(256) k_classify(MeshArgs a, uint32_t nseg)
 	const uint32_t seg = blk % nseg;
 	const uint32_t cy = (blk / nseg) % a.cby, cz = blk / (nseg * a.cby);
 	const uint32_t t0 = seg * kSegWords * 8;
-	for (uint32_t q = threadIdx.x; q < kTiles * 8; q += blockDim.x)
+	// (launched with 256 threads: fixed trip counts, so that every thread's
+	// sign loads are issued before the first wait)
+#pragma unroll
+	for (uint32_t q = threadIdx.x; q < kTiles * 8; q += 256)
 		corner_masks(a, t0 + (q >> 3), cy, cz, q & 7, all_s[q], any_s[q]);
 	__syncthreads();
 	uint8_t* bytes = reinterpret_cast<uint8_t*>(lds);
-	for (uint32_t q = threadIdx.x; q < 512; q += blockDim.x)
+#pragma unroll
+	for (uint32_t q = threadIdx.x; q < 512; q += 256)
 	{
 		const uint32_t kk = q & 7, lt = q >> 3;
 		const uint64_t all0 = all_s[q], any0 = any_s[q], all1 = all_s[q + 8], any1 = any_s[q + 8];
@@ -132,7 +136,8 @@ __global__ void __launch_bounds__(256) k_classify(MeshArgs a, uint32_t nseg)
 		for (uint32_t jj = 0; jj < 8; ++jj) bytes[(kk * 8 + jj) * 64 + lt] = uint8_t(act >> (8 * jj));
 	}
 	__syncthreads();
-	for (uint32_t o = threadIdx.x; o < 512; o += blockDim.x)
+#pragma unroll
+	for (uint32_t o = threadIdx.x; o < 512; o += 256)
 	{
 		const uint32_t ww = o & 7, jj = (o >> 3) & 7, kk = o >> 6;
 		const uint32_t w = seg * kSegWords + ww, j = cy * 8 + jj, k = cz * 8 + kk;
