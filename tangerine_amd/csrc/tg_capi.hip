@@ -1,3 +1,4 @@
+#include <atomic>
 // C ABI of the MI355X CSG distance path (declared in include/tangerine_gpu.h).
 
 #include <hip/hip_runtime.h>
@@ -299,7 +300,9 @@ int tg::upload_program(const DecodedProgram& d, tg_program** out)
 {
 	int rc = ensure_device();
 	if (rc) return rc;
+	static std::atomic<uint64_t> next_uid{ 1 };
 	auto p = std::make_unique<tg_program>();
+	p->uid = next_uid++;
 	p->decoded = d;
 	rc = p->device.ensure(d.instrs.size() * sizeof(Instr));
 	if (!rc) rc = p->entries.ensure(std::max<size_t>(d.entries.size(), 1) * sizeof(HostEntry));

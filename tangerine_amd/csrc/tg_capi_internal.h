@@ -31,6 +31,7 @@ struct DevBuf
 
 struct tg_program
 {
+	uint64_t uid = 0;    // unique per uploaded program (caches of derived device data key on it)
 	tg::DecodedProgram decoded;
 	tg::DevBuf device;   // Instr records
 	tg::DevBuf entries;  // whole program as pruning entries
@@ -66,6 +67,8 @@ struct CellPrograms
 	bool valid = false;       // false: the program is not prunable here
 	bool reran = false;       // the last pruning emitted again after a pool overflow (desc changed)
 	int emission = 0;         // after level-parallel passes: 0 automatic, 1 sequential walk, 2 a wave per cell
+	uint64_t levels_uid = 0;  // program whose entries by height `levels` holds (0: none)
+	size_t levels_bytes = 0;  // ... in the allocation of this size (a grown buffer is a new one)
 	PruneState state;
 	unsigned long long* host = nullptr;  // pinned copy of the pass counters
 	void release()
@@ -78,6 +81,7 @@ struct CellPrograms
 		scratch.release();
 		values.release();
 		levels.release();
+		levels_uid = 0;
 		lens.release();
 	}
 };

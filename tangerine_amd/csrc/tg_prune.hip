@@ -242,9 +242,16 @@ int prune_prepare(hipStream_t stream, const tg_program* p, const float origin[3]
 		if (!rc) rc = out.levels.ensure(level_order.size() * 4);
 		if (!rc) rc = out.lens.ensure(ps.waves * 64 * 4);
 		if (rc) return rc;
-		hipError_t e = hipMemcpyAsync(out.levels.ptr, level_order.data(), level_order.size() * 4,
-			hipMemcpyHostToDevice, stream);
-		if (e != hipSuccess) return hip_fail(e, "level upload");
+		// the entries by height depend on the program alone: uploaded once per
+		// program (and buffer)
+		if (out.levels_uid != p->uid || out.levels_bytes != out.levels.bytes)
+		{
+			hipError_t e = hipMemcpyAsync(out.levels.ptr, level_order.data(), level_order.size() * 4,
+				hipMemcpyHostToDevice, stream);
+			if (e != hipSuccess) return hip_fail(e, "level upload");
+			out.levels_uid = p->uid;
+			out.levels_bytes = out.levels.bytes;
+		}
 	}
 	// First guess: a quarter of the whole program per cell, at most 1 GiB; grown
 	// to the exact total and rerun if any cell did not fit (prune_cells_finish).
